@@ -1,0 +1,229 @@
+// Device-side data structures and numerics of the PLBFS solver (gfx950).
+//
+// Numerics follow the reference's CPU solver (src/Solver.cpp); the
+// per-face equilibrium-flux evaluation is reorganised for registers:
+// instead of materialising the 15 lattice populations (src/Solver.cpp:286-347,
+// 655-837), each lattice point is reconstructed, its equilibrium evaluated and
+// folded into running moments S0 = sum feq, S1 = sum e feq, Pi = sum e e^T feq.
+// With f_k = feq_k(I) - tau (feq_k(I) - feq_k) and the D3Q15 isotropy
+// sum w e_a e_b = I/3, sum w e_a e_b e_c e_d = (d_ab d_cd + d_ac d_bd + d_ad d_bc)/9,
+// the reference's flux sums are exactly
+//   sum_k e_k f_k       = (1 - tau) rho_I u_I + tau S1      (= S1, since rho_I u_I = S1)
+//   sum_k e_k e_k^T f_k = (1 - tau) rho_I (I/3 + u_I u_I^T / gamma) + tau Pi
+// so only 10 running sums live in registers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace plbfs {
+
+struct DevPatch {
+    int32_t rho_type, vel_type, is_empty, pad;
+    double rho, u, v, w;
+};
+
+// Step control block, written by the reduction kernel, read by every kernel.
+struct Ctrl {
+    int32_t done;    // nonzero: skip all further work
+    int32_t status;  // 0 ok, PLBFS_ERR_NAN, PLBFS_ERR_RHO, PLBFS_ERR_IBM
+    int32_t step;    // steps completed in this run
+    int32_t converged;
+    double res[4];
+};
+
+struct IbmDev {
+    int n_nodes;
+    int kernel;  // 0 hat2, 1 peskin4
+    double h, inv_h, dt, stiffness, node_area, margin;
+    double *x, *x0, *xref, *vel, *force;  // [3][n] component-major
+    double *norm;                           // [n] normaliser of the spreading weights
+    const int32_t *np_off, *np_cell;        // node-major pairs (candidates)
+    int64_t n_pairs;
+    const int32_t *cp_off, *cp_pair;        // affected cell -> pair ids (cell-major)
+    const int32_t *cp_cell;                 // affected cell ids
+    int n_affected;
+    const double *cell_c;                   // [3][N] centroids, new order
+    const double *vol;                      // [N]
+};
+
+template <typename S>
+struct DevMesh {
+    int N, G, n_blocks, max_items;
+    int64_t E;
+    const uint8_t *st_cnt;
+    const int32_t *st_nbr;
+    const S *st_w;
+    const double *vol, *inv_vol, *cfl_area;
+    const int32_t *blk_item_off, *it_own, *it_nbr;
+    const S *it_geo;
+    const int32_t *g_off, *g_ent;
+    const int32_t *b_patch;
+    const S *b_dmag;
+    const DevPatch *patches;
+};
+
+struct DevParams {
+    double visc, gamma, invgam, visc3_invgam, force, cfl, ceff, ref_rho, ref_mom, tol;
+    int time_stepping;
+    int use_body_force;
+};
+
+// ---------------------------------------------------------------------------
+// Ghost cells (src/Solution.cpp:759-833, src/gradients.cpp:395-504,592-648)
+// ---------------------------------------------------------------------------
+
+template <typename T>
+__device__ __forceinline__ void ghost_state(const DevPatch &P, const T po[4], T pg[4]) {
+    const int rt = P.rho_type, vt = P.vel_type;
+    if (rt == 1) pg[0] = (T)P.rho - (po[0] - (T)P.rho);
+    else if (rt == 6 || rt == 7) pg[0] = (T)P.rho;
+    else pg[0] = po[0];  // 2 (neumann, dx = 0) and 8 (symmetry)
+    if (vt == 1) {
+        pg[1] = (T)P.u - (po[1] - (T)P.u);
+        pg[2] = (T)P.v - (po[2] - (T)P.v);
+        pg[3] = (T)P.w - (po[3] - (T)P.w);
+    } else if (vt == 6) {
+        pg[1] = (T)P.u;
+        pg[2] = (T)P.v;
+        pg[3] = (T)P.w;
+    } else if (vt == 8) {
+        pg[1] = po[1];
+        pg[2] = -po[2];
+        pg[3] = po[3];
+    } else {
+        pg[1] = po[1];
+        pg[2] = po[2];
+        pg[3] = po[3];
+    }
+}
+
+// Ghost gradient from the owner's state and gradient.  rho_type 8 leaves the
+// reference's zero-initialised rho slot untouched, so it is zero here too.
+template <typename T>
+__device__ __forceinline__ void ghost_grad(const DevPatch &P, const T n[3], T dmag, const T po[4], const T go[12],
+                                           T gg[12]) {
+    const int rt = P.rho_type, vt = P.vel_type;
+    if (rt == 1) {
+        const T q = (T)2 * ((T)P.rho - po[0]) / dmag;
+        gg[0] = q * n[0];
+        gg[1] = q * n[1];
+        gg[2] = q * n[2];
+    } else if (rt == 2) {
+        gg[0] = go[0] * ((T)1 - n[0]);
+        gg[1] = go[1] * ((T)1 - n[1]);
+        gg[2] = go[2] * ((T)1 - n[2]);
+    } else if (rt == 7) {
+        gg[0] = go[0];
+        gg[1] = go[1];
+        gg[2] = go[2];
+    } else {
+        gg[0] = gg[1] = gg[2] = (T)0;
+    }
+    if (vt == 1 || vt == 8) {
+        const T qu = (T)2 * ((T)P.u - po[1]) / dmag;
+        const T qv = (T)2 * ((T)P.v - po[2]) / dmag;
+        const T qw = (T)2 * ((T)P.w - po[3]) / dmag;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            gg[3 + c] = qu * n[c];
+            gg[6 + c] = qv * n[c];
+            gg[9 + c] = qw * n[c];
+        }
+    } else if (vt == 2) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            gg[3 + c] = go[3 + c] * ((T)1 - n[c]);
+            gg[6 + c] = go[6 + c] * ((T)1 - n[c]);
+            gg[9 + c] = go[9 + c] * ((T)1 - n[c]);
+        }
+    } else {
+#pragma unroll
+        for (int q = 3; q < 12; q++) gg[q] = (T)0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Lattice evaluation (src/Solver.cpp:840-1374 restructured as moments)
+// ---------------------------------------------------------------------------
+
+template <typename T>
+struct Moments {
+    T s0, sx, sy, sz, pxx, pyy, pzz, pxy, pxz, pyz;
+};
+
+template <int EX, int EY, int EZ, typename T>
+__device__ __forceinline__ void add_pop(Moments<T> &M, T wk, T r, T u, T v, T w, T invgam) {
+    const T eu = (T)EX * u + (T)EY * v + (T)EZ * w;
+    const T usq = u * u + v * v + w * w;
+    const T f = wk * r * ((T)1 + (T)3 * eu + ((T)4.5 * eu * eu - (T)1.5 * usq) * invgam);
+    M.s0 += f;
+    if (EX) M.sx += (T)EX * f;
+    if (EY) M.sy += (T)EY * f;
+    if (EZ) M.sz += (T)EZ * f;
+    if (EX) M.pxx += f;
+    if (EY) M.pyy += f;
+    if (EZ) M.pzz += f;
+    if (EX * EY) M.pxy += (T)(EX * EY) * f;
+    if (EX * EZ) M.pxz += (T)(EX * EZ) * f;
+    if (EY * EZ) M.pyz += (T)(EY * EZ) * f;
+}
+
+// Lattice pair +e / -e.  Point +e takes the neighbour side iff e.n < 0 and
+// point -e iff e.n > 0 (the reference's "sum of signed normal components > 0"
+// tests, evaluated in x,y,z order so the sign decision is bit-identical).
+template <int EX, int EY, int EZ, typename T>
+__device__ __forceinline__ void lattice_pair(Moments<T> &M, T wk, const T n[3], T dt, const T pI[4], const T gO[12],
+                                             const T pN[4], const T gN[12], T invgam) {
+    T c = (T)0;
+    if (EX) c = c + (T)EX * n[0];
+    if (EY) c = c + (T)EY * n[1];
+    if (EZ) c = c + (T)EZ * n[2];
+    T a[4], b[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        T po = (T)0, pn = (T)0;
+        if (EX) { po += (T)EX * gO[3 * q]; pn += (T)EX * gN[3 * q]; }
+        if (EY) { po += (T)EY * gO[3 * q + 1]; pn += (T)EY * gN[3 * q + 1]; }
+        if (EZ) { po += (T)EZ * gO[3 * q + 2]; pn += (T)EZ * gN[3 * q + 2]; }
+        const bool up = c < (T)0;   // +e point from the neighbour side
+        const bool dn = c > (T)0;   // -e point from the neighbour side
+        a[q] = up ? (pN[q] - pn * dt) : (pI[q] - po * dt);
+        b[q] = dn ? (pN[q] + pn * dt) : (pI[q] + po * dt);
+    }
+    add_pop<EX, EY, EZ, T>(M, wk, a[0], a[1], a[2], a[3], invgam);
+    add_pop<-EX, -EY, -EZ, T>(M, wk, b[0], b[1], b[2], b[3], invgam);
+}
+
+// Flux through a face of area A and unit normal n (owner -> neighbour).
+template <typename T>
+__device__ __forceinline__ void lattice_flux(const T n[3], T A, T dt, T tau, const T p0[4], const T pI[4],
+                                             const T gO[12], const T pN[4], const T gN[12], T invgam, T F[4]) {
+    Moments<T> M = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    const T w0 = (T)(2.0 / 9.0), w1 = (T)(1.0 / 9.0), w7 = (T)(1.0 / 72.0);
+    add_pop<0, 0, 0, T>(M, w0, p0[0], p0[1], p0[2], p0[3], invgam);
+    lattice_pair<1, 0, 0, T>(M, w1, n, dt, pI, gO, pN, gN, invgam);
+    lattice_pair<0, 1, 0, T>(M, w1, n, dt, pI, gO, pN, gN, invgam);
+    lattice_pair<0, 0, 1, T>(M, w1, n, dt, pI, gO, pN, gN, invgam);
+    lattice_pair<1, 1, 1, T>(M, w7, n, dt, pI, gO, pN, gN, invgam);
+    lattice_pair<1, 1, -1, T>(M, w7, n, dt, pI, gO, pN, gN, invgam);
+    lattice_pair<1, -1, 1, T>(M, w7, n, dt, pI, gO, pN, gN, invgam);
+    lattice_pair<-1, 1, 1, T>(M, w7, n, dt, pI, gO, pN, gN, invgam);
+    const T rI = M.s0;
+    const T ir = (T)1 / rI;
+    const T ux = M.sx * ir, uy = M.sy * ir, uz = M.sz * ir;
+    const T om = (T)1 - tau;
+    const T third = (T)(1.0 / 3.0);
+    const T k = om * rI;
+    const T mxx = k * (third + ux * ux * invgam) + tau * M.pxx;
+    const T myy = k * (third + uy * uy * invgam) + tau * M.pyy;
+    const T mzz = k * (third + uz * uz * invgam) + tau * M.pzz;
+    const T mxy = k * (ux * uy * invgam) + tau * M.pxy;
+    const T mxz = k * (ux * uz * invgam) + tau * M.pxz;
+    const T myz = k * (uy * uz * invgam) + tau * M.pyz;
+    F[0] = (M.sx * n[0] + M.sy * n[1] + M.sz * n[2]) * A;
+    F[1] = (mxx * n[0] + mxy * n[1] + mxz * n[2]) * A;
+    F[2] = (mxy * n[0] + myy * n[1] + myz * n[2]) * A;
+    F[3] = (mxz * n[0] + myz * n[1] + mzz * n[2]) * A;
+}
+
+}  // namespace plbfs

@@ -1,0 +1,916 @@
+// PLBFS solver on one MI355X (gfx950): kernels, context and the C-ABI of
+// include/plbfs_hip.h.
+//
+// Per RK stage two kernels run back to back on the context's stream:
+//   k_grad         one thread per cell: least-squares gradients of rho,u,v,w
+//                  from the folded stencil weights (src/gradients.cpp:319-389);
+//                  ghost neighbours are evaluated from the owner and its patch.
+//   k_flux_update  one block per 256 Hilbert-consecutive cells: every face of the
+//                  block is evaluated once into an LDS slot (border faces by both
+//                  blocks), then each cell gathers its faces' slots in the
+//                  reference face order and applies the RK stage update
+//                  (src/Solver.cpp:286-438).  The last stage also writes the new
+//                  solution, the next local time step (src/Solver.cpp:1693-1776)
+//                  and per-block residual partial sums.
+// After the fourth stage k_finalize sums the partials in a fixed order
+// (src/residuals.cpp:35-44,139-147) and sets the stop flag on convergence,
+// NaN or rho blow-up; every kernel returns immediately once the flag is set.
+// No float atomics are used anywhere, so runs are bitwise reproducible.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../host/host_common.hpp"
+#include "../host/layout.hpp"
+#include "lbfs_device.hpp"
+#include "plbfs_hip.h"
+
+using namespace plbfs;
+
+#define HIP_TRY(expr)                                                                               \
+    do {                                                                                            \
+        hipError_t e_ = (expr);                                                                     \
+        if (e_ != hipSuccess)                                                                       \
+            return plbfs_fail(PLBFS_ERR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                              __FILE__, __LINE__);                                                  \
+    } while (0)
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kFinalizeThreads = 256;
+
+// ---------------------------------------------------------------------------
+// Kernels
+// ---------------------------------------------------------------------------
+
+template <typename S>
+__global__ __launch_bounds__(kBlock) void k_grad(DevMesh<S> m, const Ctrl *__restrict__ ctrl,
+                                                 const S *__restrict__ st, S *__restrict__ grad) {
+    if (ctrl->done) return;
+    const int N = m.N;
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= N) return;
+    const S *rho = st, *u = st + N, *v = st + 2 * (size_t)N, *w = st + 3 * (size_t)N;
+    const S p0[4] = {rho[i], u[i], v[i], w[i]};
+    S g[12];
+#pragma unroll
+    for (int q = 0; q < 12; q++) g[q] = (S)0;
+    const int cnt = m.st_cnt[i];
+#pragma unroll
+    for (int s = 0; s < kMaxSlots; s++) {
+        if (s < cnt) {
+            const int j = m.st_nbr[(size_t)s * N + i];
+            S pj[4];
+            if (j >= 0) {
+                pj[0] = rho[j];
+                pj[1] = u[j];
+                pj[2] = v[j];
+                pj[3] = w[j];
+            } else {
+                const DevPatch P = m.patches[m.b_patch[-1 - j]];
+                ghost_state<S>(P, p0, pj);
+            }
+            const S wx = m.st_w[((size_t)s * 3 + 0) * N + i];
+            const S wy = m.st_w[((size_t)s * 3 + 1) * N + i];
+            const S wz = m.st_w[((size_t)s * 3 + 2) * N + i];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const S d = pj[q] - p0[q];
+                g[3 * q] += wx * d;
+                g[3 * q + 1] += wy * d;
+                g[3 * q + 2] += wz * d;
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 12; q++) grad[(size_t)q * N + i] = g[q];
+}
+
+template <typename S>
+__device__ __forceinline__ void load_side(const S *__restrict__ st, const S *__restrict__ grad, int N, int c, S p[4],
+                                          S g[12]) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) p[q] = st[(size_t)q * N + c];
+#pragma unroll
+    for (int q = 0; q < 12; q++) g[q] = grad[(size_t)q * N + c];
+}
+
+// Flux of face work item e (src/Solver.cpp:286-347 with populate_lattice_macros :840-1274).
+template <typename S>
+__device__ __forceinline__ void item_flux(const DevMesh<S> &m, const DevParams &prm, int64_t e,
+                                          const S *__restrict__ st, const S *__restrict__ grad, S F[4]) {
+    const int N = m.N;
+    const int64_t E = m.E;
+    const int o = m.it_own[e];
+    const int nb = m.it_nbr[e];
+    S geo[kGeoPerItem];
+#pragma unroll
+    for (int q = 0; q < kGeoPerItem; q++) geo[q] = m.it_geo[(size_t)q * E + e];
+    const S n[3] = {geo[0], geo[1], geo[2]};
+    const S A = geo[3], dt = geo[4];
+    S po[4], go[12];
+    load_side<S>(st, grad, N, o, po, go);
+    S pI[4], pN[4], p0[4], gN[12];
+    if (nb >= 0 || nb == -1) {
+        // interior branch: a real neighbour, or the ghost whose index equals
+        // n_cells (the reference's strict "neighbour > n_cells" test, Solver.cpp:861)
+        S pn[4];
+        if (nb >= 0) {
+            load_side<S>(st, grad, N, nb, pn, gN);
+        } else {
+            const DevPatch P = m.patches[m.b_patch[0]];
+            ghost_state<S>(P, po, pn);
+            ghost_grad<S>(P, n, m.b_dmag[0], po, go, gN);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            pI[q] = po[q] - (geo[5] * go[3 * q] + geo[6] * go[3 * q + 1] + geo[7] * go[3 * q + 2]);
+            pN[q] = pn[q] - (geo[8] * gN[3 * q] + geo[9] * gN[3 * q + 1] + geo[10] * gN[3 * q + 2]);
+            p0[q] = (pI[q] + pN[q]) * (S)0.5;
+        }
+    } else {
+        const int b = -1 - nb;
+        const DevPatch P = m.patches[m.b_patch[b]];
+        ghost_grad<S>(P, n, m.b_dmag[b], po, go, gN);
+        S fv[4];
+        if (P.rho_type == 1) fv[0] = (S)P.rho;
+        else if (P.rho_type == 8) fv[0] = po[0];
+        else fv[0] = po[0] - (geo[5] * gN[0] + geo[6] * gN[1] + geo[7] * gN[2]);
+        if (P.vel_type == 1) {
+            fv[1] = (S)P.u;
+            fv[2] = (S)P.v;
+            fv[3] = (S)P.w;
+        } else if (P.vel_type == 8) {
+            fv[1] = po[1];
+            fv[2] = (S)0;
+            fv[3] = po[3];
+        } else {
+#pragma unroll
+            for (int q = 1; q < 4; q++) fv[q] = po[q] - (geo[5] * gN[3 * q] + geo[6] * gN[3 * q + 1] + geo[7] * gN[3 * q + 2]);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) pI[q] = pN[q] = p0[q] = fv[q];
+    }
+    const S tau = (S)prm.visc3_invgam / dt;
+    lattice_flux<S>(n, A, dt, tau, p0, pI, go, pN, gN, (S)prm.invgam, F);
+}
+
+__device__ __forceinline__ double local_dt(const DevParams &prm, double rho, double u, double ax, double vol) {
+    // Solver.cpp:1715-1744 with the x-slot-only CFL areas (A_y = A_z = 0)
+    double eig = (fabs(u) + prm.ceff) * ax;
+    eig = eig / prm.gamma;
+    double ve = 2 * prm.visc / prm.gamma / rho / vol;
+    ve = ve * (ax * ax);
+    eig = eig + 4 * ve;
+    return prm.cfl * vol / eig;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T x) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    return x;
+}
+
+struct StageArgs {
+    double alpha, beta;
+    int stage;  // 0..3
+};
+
+template <typename S>
+__global__ __launch_bounds__(kBlock) void k_flux_update(DevMesh<S> m, DevParams prm, StageArgs sa,
+                                                        const Ctrl *__restrict__ ctrl, const S *__restrict__ st_in,
+                                                        const S *__restrict__ grad, S *__restrict__ st_out,
+                                                        double *__restrict__ t0, double *__restrict__ t1,
+                                                        double *__restrict__ dtl, const double *__restrict__ fbody,
+                                                        double *__restrict__ partials) {
+    extern __shared__ double lds[];
+    if (ctrl->done) return;
+    const int N = m.N;
+    const int blk = blockIdx.x;
+    const int e0 = m.blk_item_off[blk], e1 = m.blk_item_off[blk + 1];
+    const int nslot = m.max_items;
+    S *sF = reinterpret_cast<S *>(lds);
+    for (int e = e0 + threadIdx.x; e < e1; e += kBlock) {
+        S F[4];
+        item_flux<S>(m, prm, e, st_in, grad, F);
+        const int sl = e - e0;
+#pragma unroll
+        for (int q = 0; q < 4; q++) sF[q * nslot + sl] = F[q];
+    }
+    __syncthreads();
+    const int i = blk * kBlock + threadIdx.x;
+    double sq[4] = {0, 0, 0, 0};
+    int err = 0;
+    if (i < N) {
+        const double iv = m.inv_vol[i];
+        double R[4] = {0, 0, 0, 0};
+        const int k0 = m.g_off[i], k1 = m.g_off[i + 1];
+        for (int k = k0; k < k1; k++) {
+            const int ent = m.g_ent[k];
+            const int sl = ent >> 1;
+            if (ent & 1) {
+#pragma unroll
+                for (int q = 0; q < 4; q++) R[q] = R[q] + (-(double)sF[q * nslot + sl] * iv);
+            } else {
+#pragma unroll
+                for (int q = 0; q < 4; q++) R[q] = R[q] + ((double)sF[q * nslot + sl] * iv);
+            }
+        }
+        if (prm.use_body_force) {
+            R[1] += fbody[i];
+            R[2] += fbody[(size_t)N + i];
+            R[3] += fbody[2 * (size_t)N + i];
+        }
+        const double dti = dtl[i];
+        const double a0 = t0[i], a1 = t0[(size_t)N + i], a2 = t0[2 * (size_t)N + i], a3 = t0[3 * (size_t)N + i];
+        // Solver.cpp:412-416
+        double f1 = a0 + R[0] * dti * sa.alpha;
+        double f2 = a1 + (R[1] + prm.force) * dti * sa.alpha;
+        double f3 = a2 + R[2] * dti * sa.alpha;
+        double f4 = a3 + R[3] * dti * sa.alpha;
+        f2 = f2 / f1;
+        f3 = f3 / f1;
+        f4 = f4 / f1;
+        // Solver.cpp:424-428
+        double b0, b1, b2, b3;
+        if (sa.stage == 0) {
+            b0 = a0; b1 = a1; b2 = a2; b3 = a3;
+        } else {
+            b0 = t1[i]; b1 = t1[(size_t)N + i]; b2 = t1[2 * (size_t)N + i]; b3 = t1[3 * (size_t)N + i];
+        }
+        const double db = dti * sa.beta;
+        b0 = b0 + db * R[0];
+        b1 = b1 + db * (R[1] + prm.force);
+        b2 = b2 + db * R[2];
+        b3 = b3 + db * R[3];
+        if (sa.stage < 3) {
+            st_out[i] = (S)f1;
+            st_out[(size_t)N + i] = (S)f2;
+            st_out[2 * (size_t)N + i] = (S)f3;
+            st_out[3 * (size_t)N + i] = (S)f4;
+            t1[i] = b0; t1[(size_t)N + i] = b1; t1[2 * (size_t)N + i] = b2; t1[3 * (size_t)N + i] = b3;
+        } else {
+            // soln = t1 as primitives (Solver.cpp:431-436); it is the next step's temp
+            const double s0 = b0, s1 = b1 / b0, s2 = b2 / b0, s3 = b3 / b0;
+            st_out[i] = (S)s0;
+            st_out[(size_t)N + i] = (S)s1;
+            st_out[2 * (size_t)N + i] = (S)s2;
+            st_out[3 * (size_t)N + i] = (S)s3;
+            t0[i] = b0; t0[(size_t)N + i] = b1; t0[2 * (size_t)N + i] = b2; t0[3 * (size_t)N + i] = b3;
+            if (prm.time_stepping == PLBFS_DT_CONSTANT) dtl[i] = prm.cfl;
+            else dtl[i] = local_dt(prm, s0, s1, m.cfl_area[i], m.vol[i]);
+            sq[0] = R[0] * R[0];
+            sq[1] = R[1] * R[1];
+            sq[2] = R[2] * R[2];
+            sq[3] = R[3] * R[3];
+            // Solver.cpp:448-493 guards on the stage state
+            if (isnan(f1) || isnan(f2)) err = 1;
+            else if (f1 / prm.ref_rho > 1000.0) err = 2;
+        }
+    }
+    if (sa.stage == 3) {
+        __shared__ double red[kBlock / 64][5];
+        const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+        double e = (double)err;
+        double r0 = wave_sum(sq[0]), r1 = wave_sum(sq[1]), r2 = wave_sum(sq[2]), r3 = wave_sum(sq[3]);
+        // error code: max over the wave
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) e = fmax(e, __shfl_xor(e, off, 64));
+        if (lane == 0) {
+            red[wv][0] = r0; red[wv][1] = r1; red[wv][2] = r2; red[wv][3] = r3; red[wv][4] = e;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double s[5] = {0, 0, 0, 0, 0};
+            for (int q = 0; q < kBlock / 64; q++) {
+                for (int c = 0; c < 4; c++) s[c] += red[q][c];
+                s[4] = fmax(s[4], red[q][4]);
+            }
+            for (int c = 0; c < 5; c++) partials[(size_t)blk * 5 + c] = s[c];
+        }
+    }
+}
+
+// Global minimum of the local time steps for the "min" and "talts" modes
+// (Solver.cpp:1746-1773): every block reduces all cells (cheap; N is read once
+// per block from L2) -- done in two kernels to keep it deterministic.
+__global__ __launch_bounds__(kBlock) void k_dt_blockmin(int N, const Ctrl *__restrict__ ctrl,
+                                                        const double *__restrict__ dtl, double *__restrict__ bmin) {
+    if (ctrl->done) return;
+    __shared__ double red[kBlock / 64];
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    double v = (i < N) ? dtl[i] : 1e300;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = red[0];
+        for (int q = 1; q < kBlock / 64; q++) s = fmin(s, red[q]);
+        bmin[blockIdx.x] = s;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_dt_apply(int N, int nblk, int mode, const Ctrl *__restrict__ ctrl,
+                                                     const double *__restrict__ bmin, double *__restrict__ dtl) {
+    if (ctrl->done) return;
+    __shared__ double smin;
+    if (threadIdx.x == 0) {
+        double s = 100000000000.0;
+        for (int q = 0; q < nblk; q++) s = (bmin[q] < s) ? bmin[q] : s;
+        smin = s;
+    }
+    __syncthreads();
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= N) return;
+    const double mn = smin;
+    if (mode == PLBFS_DT_MIN) dtl[i] = mn;
+    else dtl[i] = mn * pow(2.0, floor(log2(dtl[i] / mn)));
+}
+
+__global__ __launch_bounds__(kBlock) void k_init_dt(int N, DevParams prm, const Ctrl *__restrict__ ctrl,
+                                                    const double *__restrict__ cfl_area,
+                                                    const double *__restrict__ vol, const double *__restrict__ st,
+                                                    double *__restrict__ dtl) {
+    if (ctrl->done) return;
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= N) return;
+    if (prm.time_stepping == PLBFS_DT_CONSTANT) dtl[i] = prm.cfl;
+    else dtl[i] = local_dt(prm, st[i], st[(size_t)N + i], cfl_area[i], vol[i]);
+}
+
+__global__ __launch_bounds__(kFinalizeThreads) void k_finalize(int nblk, double n_cells, double ref_rho,
+                                                               double ref_mom, double tol, Ctrl *__restrict__ ctrl,
+                                                               const double *__restrict__ partials,
+                                                               double *__restrict__ hist, int hist_cap) {
+    if (ctrl->done) return;
+    __shared__ double red[kFinalizeThreads][5];
+    double s[5] = {0, 0, 0, 0, 0};
+    for (int b = threadIdx.x; b < nblk; b += kFinalizeThreads) {
+        for (int c = 0; c < 4; c++) s[c] += partials[(size_t)b * 5 + c];
+        s[4] = fmax(s[4], partials[(size_t)b * 5 + 4]);
+    }
+    for (int c = 0; c < 5; c++) red[threadIdx.x][c] = s[c];
+    __syncthreads();
+    for (int off = kFinalizeThreads / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off) {
+            for (int c = 0; c < 4; c++) red[threadIdx.x][c] += red[threadIdx.x + off][c];
+            red[threadIdx.x][4] = fmax(red[threadIdx.x][4], red[threadIdx.x + off][4]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const int step = ctrl->step;
+        double r[4];
+        r[0] = sqrt(red[0][0] / n_cells) / ref_rho;
+        for (int c = 1; c < 4; c++) r[c] = sqrt(red[0][c] / n_cells) / ref_mom;
+        for (int c = 0; c < 4; c++) ctrl->res[c] = r[c];
+        if (step < hist_cap)
+            for (int c = 0; c < 4; c++) hist[(size_t)step * 4 + c] = r[c];
+        ctrl->step = step + 1;
+        const int e = (int)red[0][4];
+        if (e == 1) {
+            ctrl->status = PLBFS_ERR_NAN;
+            ctrl->done = 1;
+        } else if (e == 2) {
+            ctrl->status = PLBFS_ERR_RHO;
+            ctrl->done = 1;
+        } else {
+            const double me = fmax(fmax(r[3], r[1]), r[2]);
+            if (me < tol) {
+                ctrl->converged = 1;
+                ctrl->done = 1;
+            }
+        }
+    }
+}
+
+// One residual evaluation for diagnostics: flux gather only (stage-independent).
+template <typename S>
+__global__ __launch_bounds__(kBlock) void k_residual_only(DevMesh<S> m, DevParams prm, const S *__restrict__ st_in,
+                                                          const S *__restrict__ grad, const double *__restrict__ fbody,
+                                                          double *__restrict__ Rout) {
+    extern __shared__ double lds[];
+    const int N = m.N;
+    const int blk = blockIdx.x;
+    const int e0 = m.blk_item_off[blk], e1 = m.blk_item_off[blk + 1];
+    const int nslot = m.max_items;
+    S *sF = reinterpret_cast<S *>(lds);
+    for (int e = e0 + threadIdx.x; e < e1; e += kBlock) {
+        S F[4];
+        item_flux<S>(m, prm, e, st_in, grad, F);
+#pragma unroll
+        for (int q = 0; q < 4; q++) sF[q * nslot + (e - e0)] = F[q];
+    }
+    __syncthreads();
+    const int i = blk * kBlock + threadIdx.x;
+    if (i >= N) return;
+    const double iv = m.inv_vol[i];
+    double R[4] = {0, 0, 0, 0};
+    for (int k = m.g_off[i]; k < m.g_off[i + 1]; k++) {
+        const int ent = m.g_ent[k];
+        const int sl = ent >> 1;
+        for (int q = 0; q < 4; q++) {
+            const double v = (double)sF[q * nslot + sl] * iv;
+            R[q] = R[q] + ((ent & 1) ? -v : v);
+        }
+    }
+    if (prm.use_body_force) {
+        R[1] += fbody[i];
+        R[2] += fbody[(size_t)N + i];
+        R[3] += fbody[2 * (size_t)N + i];
+    }
+    for (int q = 0; q < 4; q++) Rout[(size_t)q * N + i] = R[q];
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Context
+// ---------------------------------------------------------------------------
+
+struct KStat {
+    double ms = 0;
+    int launches = 0;
+};
+
+struct plbfs_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    Layout L;
+    plbfs_params params{};
+    std::vector<plbfs_patch> patches;
+    DevParams prm{};
+    // device buffers (FP64 storage)
+    std::vector<void *> allocs;
+    DevMesh<double> dm{};
+    double *stA = nullptr, *stB = nullptr;  // [4][N] temp ping-pong
+    double *t0 = nullptr, *t1 = nullptr;    // [4][N]
+    double *grad = nullptr;                 // [12][N]
+    double *dtl = nullptr;                  // [N]
+    double *fbody = nullptr;                // [3][N]
+    double *partials = nullptr;             // [n_blocks*5]
+    double *bmin = nullptr;                 // [n_blocks]
+    double *hist = nullptr;
+    int hist_cap = 0;
+    Ctrl *ctrl = nullptr;
+    size_t lds_bytes = 0;
+    int total_steps = 0;
+    std::vector<double> last_hist;
+    // timing
+    bool timing = false;
+    KStat kstat[4];
+    struct EvPair {
+        int which;
+        hipEvent_t a, b;
+    };
+    std::vector<EvPair> pending;
+    // graph of one step
+    hipGraphExec_t graph_exec = nullptr;
+    bool graph_ok = false;
+    bool use_graph = true;
+};
+
+static int dalloc(plbfs_ctx *c, void **p, size_t bytes) {
+    if (bytes == 0) bytes = 8;
+    HIP_TRY(hipMalloc(p, bytes));
+    c->allocs.push_back(*p);
+    return PLBFS_OK;
+}
+
+template <typename T>
+static int upload(plbfs_ctx *c, T **dst, const std::vector<T> &src) {
+    int rc = dalloc(c, (void **)dst, src.size() * sizeof(T));
+    if (rc) return rc;
+    if (!src.empty()) HIP_TRY(hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
+    return PLBFS_OK;
+}
+
+static int release(plbfs_ctx *c) {
+    if (c->graph_exec) {
+        hipGraphExecDestroy(c->graph_exec);
+        c->graph_exec = nullptr;
+    }
+    for (void *p : c->allocs) hipFree(p);
+    c->allocs.clear();
+    if (c->stream) {
+        hipStreamDestroy(c->stream);
+        c->stream = nullptr;
+    }
+    return PLBFS_OK;
+}
+
+extern "C" int plbfs_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+extern "C" int plbfs_create(const plbfs_mesh_view *mesh, const plbfs_params *params, const plbfs_patch *patches,
+                            int32_t n_patches, const int32_t *bface_patch, plbfs_ctx **out) {
+    if (!mesh || !params || !out) return plbfs_fail(PLBFS_ERR_INPUT, "null argument");
+    *out = nullptr;
+    const plbfs_params &p = *params;
+    if (!(p.visc > 0) || !(p.gamma > 0) || !(p.cfl > 0) || !(p.max_velocity > 0) || !(p.ref_rho > 0))
+        return plbfs_fail(PLBFS_ERR_INPUT, "visc, gamma, cfl, max_velocity and ref_rho must be positive");
+    if (p.time_stepping < 0 || p.time_stepping > 3 || p.gradient_type < 0 || p.gradient_type > 2)
+        return plbfs_fail(PLBFS_ERR_INPUT, "bad time_stepping or gradient_type");
+    if (p.precision != PLBFS_FP64 && p.precision != PLBFS_FP32)
+        return plbfs_fail(PLBFS_ERR_INPUT, "bad precision %d", p.precision);
+    if (p.precision == PLBFS_FP32) return plbfs_fail(PLBFS_ERR_INPUT, "FP32 storage is built by plbfs_create32");
+    if (plbfs_device_count() <= 0) return plbfs_fail(PLBFS_ERR_NODEVICE, "no HIP device visible");
+    auto *c = new plbfs_ctx();
+    c->params = p;
+    c->patches.assign(patches, patches + n_patches);
+    int rc = build_layout(*mesh, p, patches, n_patches, bface_patch, kBlock, c->L);
+    if (rc) {
+        delete c;
+        return rc;
+    }
+    const Layout &L = c->L;
+    c->lds_bytes = (size_t)L.max_items_per_block * 4 * sizeof(double);
+    if (c->lds_bytes > 64 * 1024) {
+        delete c;
+        return plbfs_fail(PLBFS_ERR_INPUT, "a block needs %zu B of LDS for %d face slots (limit 64 KiB)",
+                          c->lds_bytes, L.max_items_per_block);
+    }
+    auto fail = [&](int r) {
+        release(c);
+        delete c;
+        return r;
+    };
+    if (hipGetDevice(&c->device) != hipSuccess) return fail(plbfs_fail(PLBFS_ERR_HIP, "hipGetDevice"));
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+        return fail(plbfs_fail(PLBFS_ERR_HIP, "hipStreamCreate"));
+    const int N = L.N;
+    DevMesh<double> &d = c->dm;
+    d.N = N;
+    d.G = L.G;
+    d.n_blocks = L.n_blocks;
+    d.max_items = L.max_items_per_block;
+    d.E = L.n_items;
+    std::vector<DevPatch> dp(n_patches);
+    for (int q = 0; q < n_patches; q++) {
+        dp[q].rho_type = patches[q].rho_type;
+        dp[q].vel_type = patches[q].vel_type;
+        dp[q].is_empty = patches[q].is_empty;
+        dp[q].pad = 0;
+        dp[q].rho = patches[q].rho;
+        dp[q].u = patches[q].u;
+        dp[q].v = patches[q].v;
+        dp[q].w = patches[q].w;
+    }
+    std::vector<int32_t> bpatch = L.b_patch;
+    if (bpatch.empty()) bpatch.push_back(0);
+    std::vector<double> bdmag = L.b_dmag;
+    if (bdmag.empty()) bdmag.push_back(1.0);
+    if ((rc = upload(c, (uint8_t **)&d.st_cnt, L.st_cnt)) || (rc = upload(c, (int32_t **)&d.st_nbr, L.st_nbr)) ||
+        (rc = upload(c, (double **)&d.st_w, L.st_w)) || (rc = upload(c, (double **)&d.vol, L.vol)) ||
+        (rc = upload(c, (double **)&d.inv_vol, L.inv_vol)) || (rc = upload(c, (double **)&d.cfl_area, L.cfl_area)) ||
+        (rc = upload(c, (int32_t **)&d.blk_item_off, L.blk_item_off)) ||
+        (rc = upload(c, (int32_t **)&d.it_own, L.it_own)) || (rc = upload(c, (int32_t **)&d.it_nbr, L.it_nbr)) ||
+        (rc = upload(c, (double **)&d.it_geo, L.it_geo)) || (rc = upload(c, (int32_t **)&d.g_off, L.g_off)) ||
+        (rc = upload(c, (int32_t **)&d.g_ent, L.g_ent)) || (rc = upload(c, (int32_t **)&d.b_patch, bpatch)) ||
+        (rc = upload(c, (double **)&d.b_dmag, bdmag)) || (rc = upload(c, (DevPatch **)&d.patches, dp)))
+        return fail(rc);
+    const size_t n4 = (size_t)4 * N * sizeof(double);
+    if ((rc = dalloc(c, (void **)&c->stA, n4)) || (rc = dalloc(c, (void **)&c->stB, n4)) ||
+        (rc = dalloc(c, (void **)&c->t0, n4)) || (rc = dalloc(c, (void **)&c->t1, n4)) ||
+        (rc = dalloc(c, (void **)&c->grad, (size_t)12 * N * sizeof(double))) ||
+        (rc = dalloc(c, (void **)&c->dtl, (size_t)N * sizeof(double))) ||
+        (rc = dalloc(c, (void **)&c->fbody, (size_t)3 * N * sizeof(double))) ||
+        (rc = dalloc(c, (void **)&c->partials, (size_t)L.n_blocks * 5 * sizeof(double))) ||
+        (rc = dalloc(c, (void **)&c->bmin, (size_t)L.n_blocks * sizeof(double))) ||
+        (rc = dalloc(c, (void **)&c->ctrl, sizeof(Ctrl))))
+        return fail(rc);
+    {
+        std::vector<double> z((size_t)3 * N, 0.0);
+        if (hipMemcpy(c->fbody, z.data(), z.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+            return fail(plbfs_fail(PLBFS_ERR_HIP, "upload body force"));
+        Ctrl h{};
+        if (hipMemcpy(c->ctrl, &h, sizeof(Ctrl), hipMemcpyHostToDevice) != hipSuccess)
+            return fail(plbfs_fail(PLBFS_ERR_HIP, "upload ctrl"));
+    }
+    DevParams &prm = c->prm;
+    prm.visc = p.visc;
+    prm.gamma = p.gamma;
+    prm.invgam = 1.0 / p.gamma;
+    prm.visc3_invgam = p.visc * 3 / p.gamma;
+    prm.force = p.force;
+    prm.cfl = p.cfl;
+    prm.ceff = p.max_velocity * std::sqrt(1 - p.gamma + std::pow(p.gamma / std::sqrt(3) / p.max_velocity, 2));
+    prm.ref_rho = p.ref_rho;
+    prm.ref_mom = p.ref_rho * p.max_velocity;
+    prm.tol = p.tolerance;
+    prm.time_stepping = p.time_stepping;
+    prm.use_body_force = 0;
+    const char *g = std::getenv("PLBFS_NO_GRAPH");
+    c->use_graph = !(g && g[0] == '1');
+    *out = c;
+    return PLBFS_OK;
+}
+
+extern "C" int plbfs_destroy(plbfs_ctx *c) {
+    if (!c) return PLBFS_OK;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    release(c);
+    delete c;
+    return PLBFS_OK;
+}
+
+static int to_device_state(plbfs_ctx *c, const double *rho, const double *u, const double *v, const double *w,
+                           double *dst) {
+    const int N = c->L.N;
+    std::vector<double> h((size_t)4 * N);
+    const double *src[4] = {rho, u, v, w};
+    for (int q = 0; q < 4; q++)
+        for (int n = 0; n < N; n++) h[(size_t)q * N + n] = src[q][c->L.perm[n]];
+    HIP_TRY(hipMemcpy(dst, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+    return PLBFS_OK;
+}
+
+static int from_device_fields(plbfs_ctx *c, const double *src, int nf, double *const *out) {
+    const int N = c->L.N;
+    std::vector<double> h((size_t)nf * N);
+    HIP_TRY(hipMemcpy(h.data(), src, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int q = 0; q < nf; q++)
+        if (out[q])
+            for (int n = 0; n < N; n++) out[q][c->L.perm[n]] = h[(size_t)q * N + n];
+    return PLBFS_OK;
+}
+
+extern "C" int plbfs_set_state(plbfs_ctx *c, const double *rho, const double *u, const double *v, const double *w) {
+    if (!c || !rho || !u || !v || !w) return plbfs_fail(PLBFS_ERR_INPUT, "null argument");
+    int rc = to_device_state(c, rho, u, v, w, c->stA);
+    if (rc) return rc;
+    // the conservative accumulator starts as a copy of the primitives (Solver.cpp:195)
+    HIP_TRY(hipMemcpy(c->t0, c->stA, (size_t)4 * c->L.N * sizeof(double), hipMemcpyDeviceToDevice));
+    c->total_steps = 0;
+    return PLBFS_OK;
+}
+
+extern "C" int plbfs_get_state(plbfs_ctx *c, double *rho, double *u, double *v, double *w) {
+    if (!c) return plbfs_fail(PLBFS_ERR_INPUT, "null argument");
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    double *outs[4] = {rho, u, v, w};
+    return from_device_fields(c, c->stA, 4, outs);
+}
+
+// ---- launches -------------------------------------------------------------
+
+// Brackets one launch with events on the context's stream (timing mode only);
+// the pairs are resolved after the run synchronises, so nothing blocks here.
+struct TimedLaunch {
+    plbfs_ctx *c;
+    int which;
+    hipEvent_t a = nullptr, b = nullptr;
+    TimedLaunch(plbfs_ctx *cc, int w) : c(cc), which(w) {
+        if (c->timing) {
+            hipEventCreate(&a);
+            hipEventCreate(&b);
+            hipEventRecord(a, c->stream);
+        }
+    }
+    ~TimedLaunch() {
+        if (c->timing) {
+            hipEventRecord(b, c->stream);
+            c->pending.push_back({which, a, b});
+        }
+    }
+};
+
+static void resolve_timing(plbfs_ctx *c) {
+    for (auto &p : c->pending) {
+        hipEventSynchronize(p.b);
+        float ms = 0;
+        hipEventElapsedTime(&ms, p.a, p.b);
+        c->kstat[p.which].ms += ms;
+        c->kstat[p.which].launches++;
+        hipEventDestroy(p.a);
+        hipEventDestroy(p.b);
+    }
+    c->pending.clear();
+}
+
+static const double kAlpha[4] = {1.0, 0.5, 0.5, 1.0};
+static const double kBeta[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
+
+static int launch_dt_modes(plbfs_ctx *c) {
+    const int N = c->L.N, nb = c->L.n_blocks;
+    if (c->params.time_stepping == PLBFS_DT_MIN || c->params.time_stepping == PLBFS_DT_TALTS) {
+        hipLaunchKernelGGL(k_dt_blockmin, dim3(nb), dim3(kBlock), 0, c->stream, N, c->ctrl, c->dtl, c->bmin);
+        hipLaunchKernelGGL(k_dt_apply, dim3(nb), dim3(kBlock), 0, c->stream, N, nb, c->params.time_stepping,
+                           c->ctrl, c->bmin, c->dtl);
+    }
+    return PLBFS_OK;
+}
+
+static int launch_step(plbfs_ctx *c) {
+    const int nb = c->L.n_blocks;
+    double *in = c->stA, *outb = c->stB;
+    for (int k = 0; k < 4; k++) {
+        {
+            TimedLaunch t(c, PLBFS_KSTAT_GRAD);
+            hipLaunchKernelGGL(k_grad<double>, dim3(nb), dim3(kBlock), 0, c->stream, c->dm, c->ctrl, in, c->grad);
+        }
+        StageArgs sa{kAlpha[k], kBeta[k], k};
+        {
+            TimedLaunch t(c, PLBFS_KSTAT_FLUX);
+            hipLaunchKernelGGL(k_flux_update<double>, dim3(nb), dim3(kBlock), c->lds_bytes, c->stream, c->dm, c->prm,
+                               sa, c->ctrl, in, c->grad, outb, c->t0, c->t1, c->dtl, c->fbody, c->partials);
+        }
+        std::swap(in, outb);
+    }
+    launch_dt_modes(c);
+    {
+        TimedLaunch t(c, PLBFS_KSTAT_REDUCE);
+        hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kFinalizeThreads), 0, c->stream, nb, (double)c->L.N,
+                           c->prm.ref_rho, c->prm.ref_mom, c->prm.tol, c->ctrl, c->partials, c->hist, c->hist_cap);
+    }
+    HIP_TRY(hipGetLastError());
+    return PLBFS_OK;
+}
+
+static int ensure_hist(plbfs_ctx *c, int steps) {
+    if (steps <= c->hist_cap) return PLBFS_OK;
+    if (c->graph_exec) {
+        hipGraphExecDestroy(c->graph_exec);
+        c->graph_exec = nullptr;
+        c->graph_ok = false;
+    }
+    int cap = std::max(steps, 1024);
+    int rc = dalloc(c, (void **)&c->hist, (size_t)cap * 4 * sizeof(double));
+    if (rc) return rc;
+    c->hist_cap = cap;
+    return PLBFS_OK;
+}
+
+static int build_graph(plbfs_ctx *c) {
+    if (c->graph_ok || !c->use_graph || c->timing) return PLBFS_OK;
+    hipGraph_t g;
+    HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int rc = launch_step(c);
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (rc) return rc;
+    if (e != hipSuccess) return plbfs_fail(PLBFS_ERR_HIP, "graph capture: %s", hipGetErrorString(e));
+    e = hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (e != hipSuccess) return plbfs_fail(PLBFS_ERR_HIP, "graph instantiate: %s", hipGetErrorString(e));
+    c->graph_ok = true;
+    return PLBFS_OK;
+}
+
+extern "C" int plbfs_run(plbfs_ctx *c, int32_t max_steps, plbfs_result *out) {
+    if (!c || max_steps < 0) return plbfs_fail(PLBFS_ERR_INPUT, "bad arguments");
+    int rc = ensure_hist(c, max_steps);
+    if (rc) return rc;
+    Ctrl h{};
+    HIP_TRY(hipMemcpyAsync(c->ctrl, &h, sizeof(Ctrl), hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_init_dt, dim3(c->L.n_blocks), dim3(kBlock), 0, c->stream, c->L.N, c->prm, c->ctrl,
+                       c->dm.cfl_area, c->dm.vol, c->stA, c->dtl);
+    launch_dt_modes(c);
+    HIP_TRY(hipGetLastError());
+    if ((rc = build_graph(c))) return rc;
+    const int interval = std::max(1, c->params.check_interval);
+    hipEvent_t ev0, ev1;
+    HIP_TRY(hipEventCreate(&ev0));
+    HIP_TRY(hipEventCreate(&ev1));
+    HIP_TRY(hipEventRecord(ev0, c->stream));
+    for (int s = 0; s < max_steps; s++) {
+        if (c->graph_ok) {
+            HIP_TRY(hipGraphLaunch(c->graph_exec, c->stream));
+        } else if ((rc = launch_step(c))) {
+            return rc;
+        }
+        if ((s + 1) % interval == 0 || s == max_steps - 1) {
+            HIP_TRY(hipMemcpyAsync(&h, c->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, c->stream));
+            HIP_TRY(hipStreamSynchronize(c->stream));
+            if (h.done) break;
+        }
+    }
+    HIP_TRY(hipEventRecord(ev1, c->stream));
+    HIP_TRY(hipEventSynchronize(ev1));
+    resolve_timing(c);
+    float ms = 0;
+    hipEventElapsedTime(&ms, ev0, ev1);
+    hipEventDestroy(ev0);
+    hipEventDestroy(ev1);
+    HIP_TRY(hipMemcpy(&h, c->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost));
+    c->last_hist.assign((size_t)h.step * 4, 0.0);
+    if (h.step > 0)
+        HIP_TRY(hipMemcpy(c->last_hist.data(), c->hist, (size_t)std::min(h.step, c->hist_cap) * 4 * sizeof(double),
+                          hipMemcpyDeviceToHost));
+    c->total_steps += h.step;
+    if (out) {
+        out->status = h.status;
+        out->steps = h.step;
+        out->converged = h.converged;
+        out->total_steps = c->total_steps;
+        for (int q = 0; q < 4; q++) out->residual[q] = h.res[q];
+        out->solve_ms = ms;
+    }
+    return h.status;
+}
+
+extern "C" int plbfs_get_residual_history(plbfs_ctx *c, double *out, int32_t max_steps) {
+    if (!c || !out) return plbfs_fail(PLBFS_ERR_INPUT, "null argument");
+    const size_t n = std::min((size_t)max_steps * 4, c->last_hist.size());
+    std::memcpy(out, c->last_hist.data(), n * sizeof(double));
+    return PLBFS_OK;
+}
+
+extern "C" int plbfs_eval_residual(plbfs_ctx *c, double *grads, double *R) {
+    if (!c) return plbfs_fail(PLBFS_ERR_INPUT, "null argument");
+    const int N = c->L.N, nb = c->L.n_blocks;
+    Ctrl h{};
+    HIP_TRY(hipMemcpyAsync(c->ctrl, &h, sizeof(Ctrl), hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_grad<double>, dim3(nb), dim3(kBlock), 0, c->stream, c->dm, c->ctrl, c->stA, c->grad);
+    double *Rd = c->stB;  // scratch: stB is rewritten before it is read in a step
+    hipLaunchKernelGGL(k_residual_only<double>, dim3(nb), dim3(kBlock), c->lds_bytes, c->stream, c->dm, c->prm,
+                       c->stA, c->grad, c->fbody, Rd);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (grads) {
+        std::vector<double> h2((size_t)12 * N);
+        HIP_TRY(hipMemcpy(h2.data(), c->grad, h2.size() * sizeof(double), hipMemcpyDeviceToHost));
+        for (int n = 0; n < N; n++)
+            for (int q = 0; q < 12; q++) grads[(size_t)c->L.perm[n] * 12 + q] = h2[(size_t)q * N + n];
+    }
+    if (R) {
+        std::vector<double> h4((size_t)4 * N);
+        HIP_TRY(hipMemcpy(h4.data(), Rd, h4.size() * sizeof(double), hipMemcpyDeviceToHost));
+        for (int n = 0; n < N; n++)
+            for (int q = 0; q < 4; q++) R[(size_t)c->L.perm[n] * 4 + q] = h4[(size_t)q * N + n];
+    }
+    return PLBFS_OK;
+}
+
+extern "C" int plbfs_get_dt(plbfs_ctx *c, double *dt) {
+    if (!c || !dt) return plbfs_fail(PLBFS_ERR_INPUT, "null argument");
+    Ctrl h{};
+    HIP_TRY(hipMemcpyAsync(c->ctrl, &h, sizeof(Ctrl), hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_init_dt, dim3(c->L.n_blocks), dim3(kBlock), 0, c->stream, c->L.N, c->prm, c->ctrl,
+                       c->dm.cfl_area, c->dm.vol, c->stA, c->dtl);
+    launch_dt_modes(c);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    double *outs[1] = {dt};
+    return from_device_fields(c, c->dtl, 1, outs);
+}
+
+extern "C" int plbfs_enable_timing(plbfs_ctx *c, int32_t on) {
+    if (!c) return plbfs_fail(PLBFS_ERR_INPUT, "null argument");
+    c->timing = on != 0;
+    for (auto &k : c->kstat) k = KStat();
+    return PLBFS_OK;
+}
+
+extern "C" int plbfs_kernel_stats(plbfs_ctx *c, int32_t which, double *total_ms, int32_t *launches) {
+    if (!c || which < 0 || which > 3) return plbfs_fail(PLBFS_ERR_INPUT, "bad arguments");
+    if (total_ms) *total_ms = c->kstat[which].ms;
+    if (launches) *launches = c->kstat[which].launches;
+    return PLBFS_OK;
+}
+
+extern "C" int plbfs_layout_info(plbfs_ctx *c, int64_t *info) {
+    if (!c || !info) return plbfs_fail(PLBFS_ERR_INPUT, "null argument");
+    const Layout &L = c->L;
+    int64_t slots = 0;
+    for (int n = 0; n < L.N; n++) slots += L.st_cnt[n];
+    info[0] = L.N;
+    info[1] = L.F;
+    info[2] = L.n_items;
+    info[3] = slots;
+    info[4] = L.G;
+    info[5] = L.n_blocks;
+    info[6] = L.max_items_per_block;
+    info[7] = (int64_t)L.g_ent.size();
+    return PLBFS_OK;
+}
+
+// IBM entry points are provided by ibm.hip; placeholders until then.
+extern "C" int plbfs_set_ibm(plbfs_ctx *c, const plbfs_ibm_nodes *nodes) {
+    (void)c;
+    (void)nodes;
+    return plbfs_fail(PLBFS_ERR_INPUT, "immersed boundary not available in this build");
+}
+extern "C" int plbfs_get_ibm(plbfs_ctx *c, double *x, double *vel, double *force) {
+    (void)c; (void)x; (void)vel; (void)force;
+    return plbfs_fail(PLBFS_ERR_INPUT, "immersed boundary not available in this build");
+}
+extern "C" int plbfs_get_body_force(plbfs_ctx *c, double *f) {
+    if (!c || !f) return plbfs_fail(PLBFS_ERR_INPUT, "null argument");
+    const int N = c->L.N;
+    std::vector<double> h((size_t)3 * N);
+    HIP_TRY(hipMemcpy(h.data(), c->fbody, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int n = 0; n < N; n++)
+        for (int q = 0; q < 3; q++) f[(size_t)c->L.perm[n] * 3 + q] = h[(size_t)q * N + n];
+    return PLBFS_OK;
+}

@@ -1,0 +1,510 @@
+// XML case layer and whole-program driver (include/plbfs_case.h).
+//
+// Parameter derivation follows src/preprocessor.cpp:27-84 (velocities scaled by
+// mach_no/scale), :187-302 (boundary types by token), :438-515 (globals and
+// initial conditions), src/global_variables.cpp:20-56 (nu, tau, preconditioned
+// gamma), src/Boundary_Conditions.cpp:388-452 (patch lookup by type token),
+// src/Solution.cpp:94-189 (initial conditions) and src/program.cpp:40-137 (the
+// run sequence).  Lagrangian objects: src/lagrangian_object.cpp:79-113 (rigid
+// cylinder); the rigid sphere with a Fibonacci node lattice is this build's.
+#include <sys/stat.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "host_common.hpp"
+#include "plbfs_case.h"
+#include "xml.hpp"
+
+struct plbfs_mesh;  // defined in mesh.cpp
+
+namespace {
+
+struct BcSpec {
+    std::string name;
+    double rho = 0, u = 0, v = 0, w = 0;  // u,v,w scaled by the mach factor
+    int vel_type = 0, rho_type = 0;
+};
+
+struct ObjSpec {
+    std::string name, type;
+    int num_nodes = 0;
+    double radius = 0, depth = 0, stiffness = 0, cx = 0, cy = 0, cz = 0;
+    int kernel = PLBFS_DELTA_HAT2;
+};
+
+int bc_code(const std::string &t) {
+    static const std::map<std::string, int> m = {{"dirichlet", 1}, {"neumann", 2},    {"periodic", 3},
+                                                 {"parabolic-N", 4}, {"parabolic-W", 5}, {"inlet", 6},
+                                                 {"wall", 7},      {"symmetry-x", 8}, {"parabolic", 9}};
+    auto it = m.find(t);
+    return it == m.end() ? 0 : it->second;
+}
+
+bool read_text(const std::string &path, std::string &out) {
+    FILE *f = std::fopen(path.c_str(), "rb");
+    if (!f) return false;
+    std::fseek(f, 0, SEEK_END);
+    long n = std::ftell(f);
+    std::fseek(f, 0, SEEK_SET);
+    out.resize(n > 0 ? (size_t)n : 0);
+    size_t got = n > 0 ? std::fread(&out[0], 1, (size_t)n, f) : 0;
+    std::fclose(f);
+    return got == out.size();
+}
+
+void mkdirs(const std::string &p) {
+    std::string cur;
+    for (size_t i = 0; i < p.size(); i++) {
+        cur += p[i];
+        if (p[i] == '/' || i + 1 == p.size()) ::mkdir(cur.c_str(), 0755);
+    }
+}
+
+}  // namespace
+
+struct plbfs_case {
+    std::map<std::string, double> num;      // "section/key" -> number
+    std::map<std::string, std::string> str;  // "section/key" -> text
+    std::vector<BcSpec> bcs;
+    std::vector<ObjSpec> objects;
+    plbfs_params params{};
+    std::string xml_path;
+
+    double n(const std::string &k, double def = 0.0) const {
+        auto it = num.find(k);
+        return it == num.end() ? def : it->second;
+    }
+    std::string s(const std::string &k, const std::string &def = "") const {
+        auto it = str.find(k);
+        return it == str.end() ? def : it->second;
+    }
+};
+
+static void flatten(const plbfs::XmlNode &node, const std::string &prefix, plbfs_case &c) {
+    for (auto &ch : node.children) {
+        const std::string key = prefix.empty() ? ch->name : prefix + "/" + ch->name;
+        if (ch->children.empty()) {
+            c.str[key] = ch->text;
+            char *end = nullptr;
+            double v = std::strtod(ch->text.c_str(), &end);
+            if (end != ch->text.c_str() && end && *end == '\0') c.num[key] = v;
+        } else if (ch->name != "bc" && ch->name != "object") {
+            flatten(*ch, key, c);
+        }
+    }
+}
+
+extern "C" int plbfs_case_load(const char *xml_path, plbfs_case **out) {
+    if (!xml_path || !out) return plbfs_fail(PLBFS_ERR_INPUT, "null argument");
+    *out = nullptr;
+    std::string text, err;
+    if (!read_text(xml_path, text)) return plbfs_fail(PLBFS_ERR_INPUT, "cannot read %s", xml_path);
+    auto root = plbfs::xml_parse(text, err);
+    if (!root) return plbfs_fail(PLBFS_ERR_INPUT, "%s: %s", xml_path, err.c_str());
+    auto *c = new plbfs_case();
+    c->xml_path = xml_path;
+    flatten(*root, "", *c);
+    // required elements (the reference dereferences each without checks)
+    const char *required[] = {"global_variables/tolerance", "global_variables/simulation_length",
+                              "global_variables/time_marching_step", "global_variables/reynolds_no",
+                              "global_variables/mach_no", "global_variables/import_file",
+                              "geometry/ref_length", "initial_conditions/average_rho"};
+    for (const char *k : required) {
+        if (!c->num.count(k) && !c->str.count(k)) {
+            delete c;
+            return plbfs_fail(PLBFS_ERR_INPUT, "%s: missing element %s", xml_path, k);
+        }
+    }
+    const double U = c->n("global_variables/mach_no");
+    const double scale = c->n("global_variables/scale", 1.0);
+    const double factor = U / scale;  // preprocessor.cpp:58,190
+    if (const plbfs::XmlNode *ub = root->child("uns_boundary_conditions")) {
+        for (auto &bc : ub->children) {
+            if (bc->name != "bc") continue;
+            BcSpec b;
+            auto t = [&](const char *k) {
+                const plbfs::XmlNode *e = bc->child(k);
+                return e ? e->text : std::string();
+            };
+            b.name = t("name");
+            b.rho = std::atof(t("rho").c_str());
+            b.u = std::atof(t("u").c_str()) * factor;
+            b.v = std::atof(t("v").c_str()) * factor;
+            b.w = std::atof(t("w").c_str()) * factor;
+            b.vel_type = bc_code(t("vel_type"));
+            b.rho_type = bc_code(t("rho_type"));
+            c->bcs.push_back(b);
+        }
+    }
+    if (const plbfs::XmlNode *lo = root->child("lagrangian_objects")) {
+        for (auto &ob : lo->children) {
+            if (ob->name != "object") continue;
+            ObjSpec o;
+            auto t = [&](const char *k) {
+                const plbfs::XmlNode *e = ob->child(k);
+                return e ? e->text : std::string();
+            };
+            o.name = t("name");
+            o.type = t("type");
+            o.num_nodes = (int)std::atof(t("num_nodes").c_str());
+            o.radius = std::atof(t("radius").c_str());
+            o.depth = std::atof(t("depth").c_str());
+            o.stiffness = std::atof(t("stiffness").c_str());
+            o.cx = std::atof(t("centre_x").c_str());
+            o.cy = std::atof(t("centre_y").c_str());
+            o.cz = std::atof(t("centre_z").c_str());
+            const std::string k = t("delta");
+            o.kernel = (k == "peskin4" || (k.empty() && o.type == "rigid_sphere")) ? PLBFS_DELTA_PESKIN4
+                                                                                    : PLBFS_DELTA_HAT2;
+            c->objects.push_back(o);
+        }
+    }
+    // global_variables::initialise (global_variables.cpp:20-56)
+    plbfs_params &p = c->params;
+    p.visc = U * c->n("geometry/ref_length") / c->n("global_variables/reynolds_no");
+    p.gamma = c->n("global_variables/pre_condition_gamma", 1.0);
+    if (c->s("global_variables/preconditioning") == "true" && std::fabs(p.gamma - 1) < 0.00001)
+        p.gamma = U * 2.0 * std::sqrt(3);
+    p.fneq_min = c->n("global_variables/fneq_min", 0.0);
+    p.cfl = c->n("global_variables/time_marching_step");
+    p.max_velocity = U;
+    p.ref_rho = c->n("initial_conditions/average_rho");
+    p.force = -c->n("initial_conditions/pressure_gradient", 0.0);
+    p.tolerance = c->n("global_variables/tolerance");
+    const std::string gc = c->s("global_variables/gradient_calc", "LS");
+    p.gradient_type = gc == "LS" ? PLBFS_GRAD_LS : (gc == "hybrid" ? PLBFS_GRAD_HYBRID : PLBFS_GRAD_GG);
+    const std::string ts = c->s("global_variables/time_stepping", "local");
+    p.time_stepping = ts == "local" ? PLBFS_DT_LOCAL
+                      : ts == "min" ? PLBFS_DT_MIN
+                      : ts == "talts" ? PLBFS_DT_TALTS
+                                      : PLBFS_DT_CONSTANT;
+    p.precision = c->s("global_variables/precision") == "fp32" ? PLBFS_FP32 : PLBFS_FP64;
+    p.check_interval = (int32_t)c->n("global_variables/check_interval", 10);
+    p.lsq_exact_det = c->s("global_variables/lsq_determinant") == "exact" ? 1 : 0;
+    *out = c;
+    return PLBFS_OK;
+}
+
+extern "C" void plbfs_case_free(plbfs_case *c) { delete c; }
+
+extern "C" int plbfs_case_params(const plbfs_case *c, plbfs_params *p) {
+    if (!c || !p) return plbfs_fail(PLBFS_ERR_INPUT, "null argument");
+    *p = c->params;
+    return PLBFS_OK;
+}
+
+extern "C" int plbfs_case_string(const plbfs_case *c, const char *key, char *buf, int32_t len) {
+    if (!c || !key || !buf || len <= 0) return plbfs_fail(PLBFS_ERR_INPUT, "bad arguments");
+    auto it = c->str.find(key);
+    if (it == c->str.end()) return plbfs_fail(PLBFS_ERR_INPUT, "no element %s", key);
+    std::snprintf(buf, (size_t)len, "%s", it->second.c_str());
+    return PLBFS_OK;
+}
+
+extern "C" double plbfs_case_number(const plbfs_case *c, const char *key) {
+    if (!c || !key) return NAN;
+    auto it = c->num.find(key);
+    return it == c->num.end() ? NAN : it->second;
+}
+
+extern "C" int plbfs_case_patches(const plbfs_case *c, const plbfs_mesh *m, plbfs_patch *out, int32_t n_patches) {
+    if (!c || !m || !out) return plbfs_fail(PLBFS_ERR_INPUT, "null argument");
+    int64_t info[6];
+    plbfs_mesh_info(m, info);
+    if (n_patches != info[4]) return plbfs_fail(PLBFS_ERR_INPUT, "mesh has %lld patches", (long long)info[4]);
+    char buf[256];
+    for (int q = 0; q < n_patches; q++) {
+        plbfs_mesh_patch_type(m, q, buf, sizeof buf);
+        const BcSpec *b = nullptr;
+        for (auto &x : c->bcs)
+            if (x.name == buf) {
+                b = &x;
+                break;
+            }
+        if (!b) return plbfs_fail(PLBFS_ERR_BCTYPE, "no <bc> with name '%s' for mesh patch %d", buf, q);
+        out[q].rho_type = b->rho_type;
+        out[q].vel_type = b->vel_type;
+        out[q].rho = b->rho;
+        out[q].u = b->u;
+        out[q].v = b->v;
+        out[q].w = b->w;
+        out[q].is_empty = std::strcmp(buf, "empty") == 0;
+    }
+    return PLBFS_OK;
+}
+
+extern "C" int plbfs_case_initial_state(const plbfs_case *c, const plbfs_mesh *m, double *rho, double *u, double *v,
+                                        double *w) {
+    if (!c || !m || !rho || !u || !v || !w) return plbfs_fail(PLBFS_ERR_INPUT, "null argument");
+    int64_t info[6];
+    plbfs_mesh_info(m, info);
+    plbfs_mesh_view view;
+    plbfs_mesh_view_get(m, &view);
+    const int64_t T = info[0] + info[5];
+    const double factor = c->n("global_variables/mach_no") / c->n("global_variables/scale", 1.0);
+    const double ox = c->n("initial_conditions/origin_location/x"), oy = c->n("initial_conditions/origin_location/y"),
+                 oz = c->n("initial_conditions/origin_location/z");
+    const double rmx = c->n("initial_conditions/rho/origin_magnitude/x"),
+                 rmy = c->n("initial_conditions/rho/origin_magnitude/y"),
+                 rmz = c->n("initial_conditions/rho/origin_magnitude/z");
+    const double rgx = c->n("initial_conditions/rho/gradient/x"), rgy = c->n("initial_conditions/rho/gradient/y"),
+                 rgz = c->n("initial_conditions/rho/gradient/z");
+    const double vmx = c->n("initial_conditions/vel/origin_magnitude/x") * factor,
+                 vmy = c->n("initial_conditions/vel/origin_magnitude/y") * factor,
+                 vmz = c->n("initial_conditions/vel/origin_magnitude/z") * factor;
+    const double vgx = c->n("initial_conditions/vel/gradient/x") * factor,
+                 vgy = c->n("initial_conditions/vel/gradient/y") * factor,
+                 vgz = c->n("initial_conditions/vel/gradient/z") * factor;
+    for (int64_t t = 0; t < T; t++) {
+        const double dx = view.cell_centroid[3 * t] - ox, dy = view.cell_centroid[3 * t + 1] - oy,
+                     dz = view.cell_centroid[3 * t + 2] - oz;
+        const double tx = rmx + rgx * dx, ty = rmy + rgy * dy, tz = rmz + rgz * dz;
+        rho[t] = std::sqrt(std::pow(tx, 2) + std::pow(ty, 2) + std::pow(tz, 2));
+        const double ux = vmx + vgx * dx, uy = vmy + vgy * dy, uz = vmz + vgz * dz;
+        u[t] = ux + uy + uz;
+        v[t] = 0.0;
+        w[t] = 0.0;
+    }
+    return PLBFS_OK;
+}
+
+extern "C" int plbfs_case_n_objects(const plbfs_case *c) { return c ? (int)c->objects.size() : 0; }
+
+extern "C" int plbfs_case_object_nodes(const plbfs_case *c, int32_t k, double *x, int32_t max_nodes, int32_t *n_out,
+                                       double *stiffness, double *node_area, int32_t *kernel) {
+    if (!c || k < 0 || k >= (int)c->objects.size()) return plbfs_fail(PLBFS_ERR_INPUT, "bad object index");
+    const ObjSpec &o = c->objects[k];
+    const double PI = 3.14159265358979323846;
+    std::vector<double> pts;
+    double area = 0;
+    if (o.type == "rigid_cylinder") {
+        // lagrangian_object.cpp:79-113; dA as in update_node_forces (immersed_boundary_method.cu:168)
+        const int depth_nodes = (int)std::sqrt((double)o.num_nodes), radial = depth_nodes;
+        const double delta_z = o.depth / depth_nodes;
+        for (int j = 0; j < depth_nodes; j++)
+            for (int i = 0; i < radial; i++) {
+                pts.push_back(o.cx + o.radius * std::sin(2 * PI * i / radial));
+                pts.push_back(o.cy + o.radius * std::cos(2 * PI * i / radial));
+                pts.push_back(o.cz + o.depth / 2 - j * o.depth / depth_nodes - delta_z / 2);
+            }
+        area = 2 * PI * o.radius / o.num_nodes * o.depth;
+    } else if (o.type == "rigid_sphere") {
+        const int n = o.num_nodes;
+        const double ga = PI * (3.0 - std::sqrt(5.0));
+        for (int i = 0; i < n; i++) {
+            const double z = 1.0 - (2.0 * i + 1.0) / n;
+            const double r = std::sqrt(std::max(0.0, 1.0 - z * z));
+            const double ph = ga * i;
+            pts.push_back(o.cx + o.radius * r * std::cos(ph));
+            pts.push_back(o.cy + o.radius * r * std::sin(ph));
+            pts.push_back(o.cz + o.radius * z);
+        }
+        area = 4 * PI * o.radius * o.radius / n;
+    } else {
+        return plbfs_fail(PLBFS_ERR_INPUT, "object '%s': unsupported type '%s'", o.name.c_str(), o.type.c_str());
+    }
+    const int n = (int)pts.size() / 3;
+    if (n_out) *n_out = n;
+    if (x) {
+        if (n > max_nodes) return plbfs_fail(PLBFS_ERR_INPUT, "object has %d nodes > %d", n, max_nodes);
+        std::memcpy(x, pts.data(), pts.size() * sizeof(double));
+    }
+    if (stiffness) *stiffness = o.stiffness;
+    if (node_area) *node_area = area;
+    if (kernel) *kernel = o.kernel;
+    return PLBFS_OK;
+}
+
+// ---- output ----------------------------------------------------------------
+
+extern "C" int plbfs_write_fields(const char *dir, int32_t n, const double *rho, const double *u, const double *v,
+                                  const double *w) {
+    if (!dir || n < 0) return plbfs_fail(PLBFS_ERR_INPUT, "bad arguments");
+    mkdirs(dir);
+    const char *names[4] = {"rho", "u", "v", "w"};
+    const double *src[4] = {rho, u, v, w};
+    for (int q = 0; q < 4; q++) {
+        std::string path = std::string(dir) + "/" + names[q] + ".txt";
+        FILE *f = std::fopen(path.c_str(), "w");
+        if (!f) return plbfs_fail(PLBFS_ERR_INPUT, "cannot write %s", path.c_str());
+        // the reference's "i ,value" lines (Solution.cpp:238-244), at full precision
+        for (int i = 0; i < n; i++) std::fprintf(f, "%d ,%.17g\n", i, src[q][i]);
+        std::fclose(f);
+    }
+    return PLBFS_OK;
+}
+
+extern "C" int plbfs_read_fields(const char *dir, int32_t n, double *rho, double *u, double *v, double *w) {
+    if (!dir || n < 0) return plbfs_fail(PLBFS_ERR_INPUT, "bad arguments");
+    const char *names[4] = {"rho", "u", "v", "w"};
+    double *dst[4] = {rho, u, v, w};
+    for (int q = 0; q < 4; q++) {
+        std::string path = std::string(dir) + "/" + names[q] + ".txt", text;
+        if (!read_text(path, text)) return plbfs_fail(PLBFS_ERR_INPUT, "cannot read %s", path.c_str());
+        int i = 0;
+        size_t pos = 0;
+        while (pos < text.size() && i < n) {
+            size_t e = text.find('\n', pos);
+            if (e == std::string::npos) e = text.size();
+            size_t comma = text.find(',', pos);
+            if (comma != std::string::npos && comma < e) dst[q][i++] = std::strtod(text.c_str() + comma + 1, nullptr);
+            pos = e + 1;
+        }
+        if (i != n) return plbfs_fail(PLBFS_ERR_INPUT, "%s has %d values, expected %d", path.c_str(), i, n);
+    }
+    return PLBFS_OK;
+}
+
+// ASCII Tecplot polyhedral zone with cell-centred variables (the content of
+// tecplot_output.cpp:273-466 and the grid writer unstructured_mesh.cpp:1990-2201,
+// without TecIO): left element = owner + 1, right = neighbour + 1 or 0.
+extern "C" int plbfs_write_tecplot(const plbfs_mesh *m, const char *path, const double *rho, const double *u,
+                                   const double *v, const double *w, double time) {
+    if (!m || !path) return plbfs_fail(PLBFS_ERR_INPUT, "null argument");
+    int64_t info[6];
+    plbfs_mesh_info(m, info);
+    const int N = (int)info[0], F = (int)info[1], NI = (int)info[2], P = (int)info[3];
+    std::vector<float> px(P), py(P), pz(P);
+    std::vector<int32_t> owner(F), nb(NI > 0 ? NI : 1), fo(F + 1), fv((size_t)plbfs_mesh_n_face_verts(m));
+    plbfs_mesh_raw(m, px.data(), py.data(), pz.data(), owner.data(), nb.data(), fo.data(), fv.data());
+    FILE *f = std::fopen(path, "w");
+    if (!f) return plbfs_fail(PLBFS_ERR_INPUT, "cannot write %s", path);
+    std::fprintf(f, "TITLE = \"PLBFS solution\"\nVARIABLES = \"X\" \"Y\" \"Z\" \"rho\" \"u\" \"v\" \"w\"\n");
+    std::fprintf(f,
+                 "ZONE T=\"t=%g\" ZONETYPE=FEPOLYHEDRON NODES=%d ELEMENTS=%d FACES=%d TOTALNUMFACENODES=%zu "
+                 "NUMCONNECTEDBOUNDARYFACES=0 TOTALNUMBOUNDARYCONNECTIONS=0 DATAPACKING=BLOCK "
+                 "VARLOCATION=([4-7]=CELLCENTERED) SOLUTIONTIME=%g\n",
+                 time, P, N, F, fv.size(), time);
+    auto col = [&](auto get, int n) {
+        for (int i = 0; i < n; i++) std::fprintf(f, "%.9g%c", (double)get(i), (i % 8 == 7 || i == n - 1) ? '\n' : ' ');
+    };
+    col([&](int i) { return px[i]; }, P);
+    col([&](int i) { return py[i]; }, P);
+    col([&](int i) { return pz[i]; }, P);
+    const double *fields[4] = {rho, u, v, w};
+    for (int q = 0; q < 4; q++) {
+        const double *a = fields[q];
+        col([&](int i) { return a ? a[i] : 0.0; }, N);
+    }
+    for (int k = 0; k < F; k++) std::fprintf(f, "%d%c", fo[k + 1] - fo[k], (k % 16 == 15 || k == F - 1) ? '\n' : ' ');
+    for (int k = 0; k < F; k++) {
+        for (int q = fo[k]; q < fo[k + 1]; q++) std::fprintf(f, "%d ", fv[q] + 1);
+        std::fputc('\n', f);
+    }
+    for (int k = 0; k < F; k++) std::fprintf(f, "%d%c", owner[k] + 1, (k % 16 == 15 || k == F - 1) ? '\n' : ' ');
+    for (int k = 0; k < F; k++)
+        std::fprintf(f, "%d%c", k < NI ? nb[k] + 1 : 0, (k % 16 == 15 || k == F - 1) ? '\n' : ' ');
+    std::fclose(f);
+    return PLBFS_OK;
+}
+
+// ---- program ---------------------------------------------------------------
+
+extern "C" int plbfs_program_run(const char *xml_path, plbfs_result *result) {
+    const auto t_start = std::chrono::steady_clock::now();
+    plbfs_case *c = nullptr;
+    int rc = plbfs_case_load(xml_path, &c);
+    if (rc) return rc;
+    std::unique_ptr<plbfs_case, void (*)(plbfs_case *)> cg(c, plbfs_case_free);
+    const plbfs_params &p = c->params;
+    if (c->s("global_variables/import_format", "openfoam") != "openfoam")
+        return plbfs_fail(PLBFS_ERR_INPUT, "only import_format openfoam is supported");
+    plbfs_mesh *m = nullptr;
+    rc = plbfs_mesh_read_openfoam(c->s("global_variables/import_file").c_str(), p.visc, p.gamma, p.fneq_min, &m);
+    if (rc) return rc;
+    std::unique_ptr<plbfs_mesh, void (*)(plbfs_mesh *)> mg(m, plbfs_mesh_free);
+    int64_t info[6];
+    plbfs_mesh_info(m, info);
+    const int N = (int)info[0], T = (int)(info[0] + info[5]), NP = (int)info[4];
+    std::vector<plbfs_patch> patches(NP);
+    if ((rc = plbfs_case_patches(c, m, patches.data(), NP))) return rc;
+    std::vector<double> rho(T), u(T), v(T), w(T);
+    std::string outdir = c->s("global_variables/output_directory", ".");
+    mkdirs(outdir);
+    if (c->s("global_variables/restart_analysis") == "true") {
+        if ((rc = plbfs_read_fields(outdir.c_str(), N, rho.data(), u.data(), v.data(), w.data()))) return rc;
+    } else if ((rc = plbfs_case_initial_state(c, m, rho.data(), u.data(), v.data(), w.data()))) {
+        return rc;
+    }
+    plbfs_mesh_view view;
+    plbfs_mesh_view_get(m, &view);
+    plbfs_ctx *ctx = nullptr;
+    if ((rc = plbfs_create(&view, &p, patches.data(), NP, plbfs_mesh_bface_patch(m), &ctx))) return rc;
+    std::unique_ptr<plbfs_ctx, int (*)(plbfs_ctx *)> xg(ctx, plbfs_destroy);
+    if ((rc = plbfs_set_state(ctx, rho.data(), u.data(), v.data(), w.data()))) return rc;
+    if (!c->objects.empty()) {
+        std::vector<double> all;
+        double stiff = 0, area = 0;
+        int kernel = 0;
+        for (int k = 0; k < (int)c->objects.size(); k++) {
+            int32_t n = 0, ker = 0;
+            double s = 0, a = 0;
+            if ((rc = plbfs_case_object_nodes(c, k, nullptr, 0, &n, &s, &a, &ker))) return rc;
+            std::vector<double> x((size_t)3 * n);
+            if ((rc = plbfs_case_object_nodes(c, k, x.data(), n, &n, &s, &a, &ker))) return rc;
+            all.insert(all.end(), x.begin(), x.end());
+            stiff = s;  // the reference applies object 0's parameters to all nodes (Solver_gpu.cu:708-709)
+            if (k == 0) {
+                area = a;
+                kernel = ker;
+            }
+        }
+        plbfs_ibm_nodes nodes{};
+        nodes.n_nodes = (int32_t)(all.size() / 3);
+        nodes.x_ref = all.data();
+        nodes.x = all.data();
+        nodes.stiffness = stiff;
+        nodes.node_area = area;
+        nodes.dt = p.cfl;
+        nodes.h = c->n("global_variables/delta_width", 2.0 * c->n("geometry/streaming_dt", 0.5));
+        nodes.kernel = kernel;
+        if ((rc = plbfs_set_ibm(ctx, &nodes))) return rc;
+    }
+    const int steps = (int)std::ceil(c->n("global_variables/simulation_length") * c->n("global_variables/scale", 1.0));
+    plbfs_result r{};
+    rc = plbfs_run(ctx, steps, &r);
+    if (rc != PLBFS_OK && rc != PLBFS_ERR_NAN && rc != PLBFS_ERR_RHO && rc != PLBFS_ERR_IBM) return rc;
+    const int run_rc = rc;
+    // residual history (error.txt columns of Solver.cpp:506-509)
+    {
+        std::vector<double> h((size_t)4 * (r.steps > 0 ? r.steps : 1));
+        plbfs_get_residual_history(ctx, h.data(), r.steps);
+        FILE *f = std::fopen((outdir + "/error.txt").c_str(), "w");
+        if (f) {
+            for (int t = 0; t < r.steps; t++) {
+                const double *q = &h[(size_t)4 * t];
+                std::fprintf(f, "%d, %.17g, %.17g, %.17g, %.17g, %.17g\n", t,
+                             std::fmax(std::fmax(q[3], q[1]), q[2]), q[0], q[1], q[2], q[3]);
+            }
+            std::fclose(f);
+        }
+    }
+    std::vector<double> sr(N), su(N), sv(N), sw(N);
+    if ((rc = plbfs_get_state(ctx, sr.data(), su.data(), sv.data(), sw.data()))) return rc;
+    if ((rc = plbfs_write_fields(outdir.c_str(), N, sr.data(), su.data(), sv.data(), sw.data()))) return rc;
+    mkdirs(outdir + "/plt");
+    const std::string plt = outdir + "/plt/" + std::to_string(r.total_steps) + ".dat";
+    if ((rc = plbfs_write_tecplot(m, plt.c_str(), sr.data(), su.data(), sv.data(), sw.data(), r.total_steps)))
+        return rc;
+    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+    if (FILE *f = std::fopen((outdir + "/globals.txt").c_str(), "w")) {
+        // program.cpp:192-234 fields that the solver determines
+        std::fprintf(f, "Runtime:%gs\nSolve:%gms\nSteps:%d\nConverged:%d\n", secs, r.solve_ms, r.total_steps,
+                     r.converged);
+        std::fprintf(f, "Tau:%.17g\nMach:%.17g\nReynolds: %.17g\n",
+                     3 * p.visc / c->n("geometry/streaming_dt", 0.5) + 0.5, p.max_velocity * std::sqrt(3),
+                     c->n("global_variables/reynolds_no"));
+        std::fclose(f);
+    }
+    if (result) *result = r;
+    return run_rc;
+}

@@ -1,0 +1,320 @@
+// Host precompute of the solver's device layout (FP64, built with
+// -ffp-contract=off so the LSQ arithmetic follows the reference's order).
+//
+// LSQ matrices: src/gradients.cpp:214-313 (pre_fill_LHS_and_RHS_matrix) and
+// :149-210 (add_non_macrovariable_contributions), including the reference's
+// determinant expression at :282-284, which is NOT the determinant of the
+// assembled matrix when LHS_xy != 0.  The gradient the reference evaluates,
+// inv * sum_j RHS_j (phi_j - phi_i) (:341-389), is folded here into one
+// 3-vector weight per stencil slot, W_j = inv * RHS_j.
+// CFL areas: src/Solver.cpp:1653-1686 (all three sums land in the x slot).
+#include "layout.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+#include "host_common.hpp"
+
+namespace plbfs {
+
+// Hilbert index via the transpose form of Skilling, "Programming the Hilbert
+// curve", AIP Conf. Proc. 707 (2004): axes -> transpose, then bit interleave.
+uint64_t hilbert_key3(uint32_t x, uint32_t y, uint32_t z, int bits) {
+    uint32_t X[3] = {x, y, z};
+    const uint32_t M = 1u << (bits - 1);
+    for (uint32_t Q = M; Q > 1; Q >>= 1) {
+        const uint32_t P = Q - 1;
+        for (int i = 0; i < 3; i++) {
+            if (X[i] & Q) {
+                X[0] ^= P;
+            } else {
+                const uint32_t t = (X[0] ^ X[i]) & P;
+                X[0] ^= t;
+                X[i] ^= t;
+            }
+        }
+    }
+    for (int i = 1; i < 3; i++) X[i] ^= X[i - 1];
+    uint32_t t = 0;
+    for (uint32_t Q = M; Q > 1; Q >>= 1)
+        if (X[2] & Q) t ^= Q - 1;
+    for (int i = 0; i < 3; i++) X[i] ^= t;
+    uint64_t key = 0;
+    for (int q = bits - 1; q >= 0; q--)
+        for (int i = 0; i < 3; i++) key = (key << 1) | ((X[i] >> q) & 1u);
+    return key;
+}
+
+static bool rho_type_ok(int t) { return t == 1 || t == 2 || t == 6 || t == 7 || t == 8; }
+static bool vel_type_ok(int t) { return t == 1 || t == 2 || t == 6 || t == 8; }
+
+int build_layout(const plbfs_mesh_view &m, const plbfs_params &p, const plbfs_patch *patches, int n_patches,
+                 const int32_t *bface_patch, int block_cells, Layout &L) {
+    const int N = m.n_cells, F = m.n_faces, NI = m.n_internal_faces, G = F - NI;
+    if (N <= 0 || F <= 0 || NI < 0 || NI > F || !m.owner || !m.neighbour || !m.cell_centroid || !m.cell_volume ||
+        !m.face_centroid || !m.face_normal || !m.face_area || !m.face_dt)
+        return plbfs_fail(PLBFS_ERR_INPUT, "incomplete mesh view");
+    if (n_patches <= 0 || !patches || (G > 0 && !bface_patch)) return plbfs_fail(PLBFS_ERR_INPUT, "missing patches");
+    for (int q = 0; q < n_patches; q++) {
+        if (!rho_type_ok(patches[q].rho_type) || !vel_type_ok(patches[q].vel_type))
+            return plbfs_fail(PLBFS_ERR_BCTYPE, "patch %d: unsupported bc types rho=%d vel=%d", q,
+                              patches[q].rho_type, patches[q].vel_type);
+    }
+    for (int b = 0; b < G; b++)
+        if (bface_patch[b] < 0 || bface_patch[b] >= n_patches)
+            return plbfs_fail(PLBFS_ERR_INPUT, "boundary face %d: patch %d out of range", b, bface_patch[b]);
+    for (int f = 0; f < F; f++) {
+        if (m.owner[f] < 0 || m.owner[f] >= N) return plbfs_fail(PLBFS_ERR_INPUT, "face %d: owner", f);
+        if (f < NI && (m.neighbour[f] < 0 || m.neighbour[f] >= N))
+            return plbfs_fail(PLBFS_ERR_INPUT, "face %d: neighbour", f);
+        if (f >= NI && m.neighbour[f] != N + f - NI)
+            return plbfs_fail(PLBFS_ERR_INPUT, "face %d: boundary neighbour must be the ghost index", f);
+    }
+    if (block_cells < 64 || block_cells > 1024 || block_cells % 64)
+        return plbfs_fail(PLBFS_ERR_INPUT, "block_cells %d", block_cells);
+    L = Layout();
+    L.N = N; L.F = F; L.NI = NI; L.G = G;
+    L.block_cells = block_cells;
+    const double *cc = m.cell_centroid;
+    const double *fc = m.face_centroid;
+    const double *fn = m.face_normal;
+
+    // stencil in file order (src/unstructured_mesh.cpp:1197-1222)
+    std::vector<int32_t> s_off(N + 1, 0), s_cell, s_face;
+    for (int f = 0; f < F; f++) {
+        s_off[m.owner[f] + 1]++;
+        if (f < NI) s_off[m.neighbour[f] + 1]++;
+    }
+    for (int i = 0; i < N; i++) {
+        if (s_off[i + 1] > kMaxSlots)
+            return plbfs_fail(PLBFS_ERR_INPUT, "cell %d has %d faces; at most %d are supported", i, s_off[i + 1],
+                              kMaxSlots);
+        s_off[i + 1] += s_off[i];
+    }
+    s_cell.resize(s_off[N]);
+    s_face.resize(s_off[N]);
+    {
+        std::vector<int32_t> fill(s_off.begin(), s_off.end() - 1);
+        for (int f = 0; f < F; f++) {
+            const int o = m.owner[f];
+            s_cell[fill[o]] = m.neighbour[f];
+            s_face[fill[o]++] = f;
+            if (f < NI) {
+                const int nb = m.neighbour[f];
+                s_cell[fill[nb]] = o;
+                s_face[fill[nb]++] = f;
+            }
+        }
+    }
+
+    // LSQ precompute (gradients.cpp:214-313)
+    L.lsq_inv9.assign((size_t)N * 9, 0.0);
+    L.lsq_rhs.assign((size_t)N * 18, 0.0);
+    L.cfl_area.assign(N, 0.0);
+    std::vector<double> cfl_file(N, 0.0);
+    for (int i = 0; i < N; i++) {
+        double xx = 0, xy = 0, xz = 0, yx = 0, yy = 0, yz = 0, zx = 0, zy = 0, zz = 0;
+        double max_area = 0.0, max_disp = 0.0;
+        const double vi = m.cell_volume[i];
+        for (int s = s_off[i]; s < s_off[i + 1]; s++) {
+            const int i1 = s_cell[s], f = s_face[s];
+            max_area = std::max(max_area, m.face_area[f]);
+            const double dx = cc[3 * i1] - cc[3 * i], dy = cc[3 * i1 + 1] - cc[3 * i + 1],
+                         dz = cc[3 * i1 + 2] - cc[3 * i + 2];
+            const double disp = std::sqrt(dx * dx + dy * dy + dz * dz);
+            max_disp = std::max(disp, max_disp);
+        }
+        const double ar = 2 * std::fabs(max_disp) * max_area / vi;
+        double beta;
+        if (p.gradient_type == PLBFS_GRAD_HYBRID) beta = std::min(1.0, (2 / ar));
+        else if (p.gradient_type == PLBFS_GRAD_LS) beta = 1.0;
+        else beta = 0.0;
+        xx = xx + 2 * (1 - beta) * vi;
+        yy = yy + 2 * (1 - beta) * vi;
+        zz = zz + 2 * (1 - beta) * vi;
+        int slot = 0;
+        for (int s = s_off[i]; s < s_off[i + 1]; s++, slot++) {
+            const int i1 = s_cell[s], f = s_face[s];
+            const double ni = fn[3 * f], nj = fn[3 * f + 1], nk = fn[3 * f + 2];
+            double dx = fc[3 * f] - cc[3 * i], dy = fc[3 * f + 1] - cc[3 * i + 1], dz = fc[3 * f + 2] - cc[3 * i + 2];
+            const double l_norm = dx * ni + dy * nj + dz * nk;
+            dx = cc[3 * i1] - cc[3 * i];
+            dy = cc[3 * i1 + 1] - cc[3 * i + 1];
+            dz = cc[3 * i1 + 2] - cc[3 * i + 2];
+            const double Ld = std::sqrt(std::pow(dx, 2.0) + std::pow(dy, 2.0) + std::pow(dz, 2.0));
+            const double sa = m.face_area[f];
+            const double L_norm = dx * ni + dy * nj + dz * nk;
+            const double alpha_w = std::pow(2 * l_norm / L_norm, 2.0);
+            const double w = alpha_w * sa / Ld;
+            xx = xx + w * dx * dx * beta; xy = xy + w * dx * dy * beta; xz = xz + w * dx * dz * beta;
+            yx = yx + w * dy * dx * beta; yy = yy + w * dy * dy * beta; yz = yz + w * dy * dz * beta;
+            zx = zx + w * dz * dx * beta; zy = zy + w * dz * dy * beta; zz = zz + w * dz * dz * beta;
+            double *r = &L.lsq_rhs[((size_t)i * 6 + slot) * 3];
+            if (m.owner[f] == i) {
+                r[0] = (beta * w * dx + 2 * (1 - beta) * 0.5 * ni * sa);
+                r[1] = (beta * w * dy + 2 * (1 - beta) * 0.5 * nj * sa);
+                r[2] = (beta * w * dz + 2 * (1 - beta) * 0.5 * nk * sa);
+            } else {
+                r[0] = (beta * w * dx + 2 * (1 - beta) * 0.5 * ni * -1 * sa);
+                r[1] = (beta * w * dy + 2 * (1 - beta) * 0.5 * nj * -1 * sa);
+                r[2] = (beta * w * dz + 2 * (1 - beta) * 0.5 * nk * -1 * sa);
+            }
+        }
+        // the reference's determinant expression (gradients.cpp:282-284)
+        double det = xx * (yy * zz - yz * zy) + xy * (yx * zz - yz - zx) + xz * (yx * zy - yy * zx);
+        if (p.lsq_exact_det) det = xx * (yy * zz - yz * zy) - xy * (yx * zz - yz * zx) + xz * (yx * zy - yy * zx);
+        const double a = xx, b = xy, c = xz, d = yx, e = yy, f = yz, g = zx, h = zy, k = zz;
+        double *q = &L.lsq_inv9[(size_t)i * 9];
+        q[0] = 1 / det * (e * k - f * h);
+        q[1] = 1 / det * (c * h - b * k);
+        q[2] = 1 / det * (b * f - c * e);
+        q[3] = 1 / det * (f * g - d * k);
+        q[4] = 1 / det * (a * k - c * g);
+        q[5] = 1 / det * (c * d - a * f);
+        q[6] = 1 / det * (d * h - e * g);
+        q[7] = 1 / det * (b * g - a * h);
+        q[8] = 1 / det * (a * e - b * d);
+        // CFL areas (Solver.cpp:1653-1686)
+        double ax = 0, ay = 0, az = 0;
+        for (int s = s_off[i]; s < s_off[i + 1]; s++) {
+            const int f2 = s_face[s];
+            ax = ax + std::fabs(fn[3 * f2] * m.face_area[f2]);
+            ay = ay + std::fabs(fn[3 * f2 + 1] * m.face_area[f2]);
+            az = az + std::fabs(fn[3 * f2 + 2] * m.face_area[f2]);
+        }
+        double v = 0.0;
+        v = v + ax / 2;
+        v = v + ay / 2;
+        v = v + az / 2;
+        cfl_file[i] = v;
+    }
+
+    // Hilbert order of interior centroids
+    L.perm.resize(N);
+    L.inv.resize(N);
+    {
+        double lo[3] = {cc[0], cc[1], cc[2]}, hi[3] = {cc[0], cc[1], cc[2]};
+        for (int i = 0; i < N; i++)
+            for (int a = 0; a < 3; a++) {
+                lo[a] = std::min(lo[a], cc[3 * i + a]);
+                hi[a] = std::max(hi[a], cc[3 * i + a]);
+            }
+        double span = std::max(hi[0] - lo[0], std::max(hi[1] - lo[1], hi[2] - lo[2]));
+        if (!(span > 0)) span = 1.0;
+        const int bits = 16;
+        const double scale = ((1u << bits) - 1) / span;
+        std::vector<std::pair<uint64_t, int32_t>> keys(N);
+        for (int i = 0; i < N; i++) {
+            uint32_t q3[3];
+            for (int a = 0; a < 3; a++) {
+                double t = (cc[3 * i + a] - lo[a]) * scale;
+                t = std::min(std::max(t, 0.0), (double)((1u << bits) - 1));
+                q3[a] = (uint32_t)t;
+            }
+            keys[i] = {hilbert_key3(q3[0], q3[1], q3[2], bits), i};
+        }
+        std::sort(keys.begin(), keys.end());
+        for (int n = 0; n < N; n++) {
+            L.perm[n] = keys[n].second;
+            L.inv[keys[n].second] = n;
+        }
+    }
+
+    // per-cell arrays in the new order, folded stencil weights
+    L.st_cnt.assign(N, 0);
+    L.st_nbr.assign((size_t)kMaxSlots * N, 0);
+    L.st_w.assign((size_t)kMaxSlots * 3 * N, 0.0);
+    L.vol.resize(N);
+    L.inv_vol.resize(N);
+    for (int n = 0; n < N; n++) {
+        const int i = L.perm[n];
+        const int cnt = s_off[i + 1] - s_off[i];
+        L.st_cnt[n] = (uint8_t)cnt;
+        const double *q = &L.lsq_inv9[(size_t)i * 9];
+        for (int s = 0; s < cnt; s++) {
+            const int i1 = s_cell[s_off[i] + s];
+            L.st_nbr[(size_t)s * N + n] = (i1 < N) ? L.inv[i1] : -1 - (i1 - N);
+            const double *r = &L.lsq_rhs[((size_t)i * 6 + s) * 3];
+            for (int c = 0; c < 3; c++)
+                L.st_w[((size_t)s * 3 + c) * N + n] = q[3 * c] * r[0] + q[3 * c + 1] * r[1] + q[3 * c + 2] * r[2];
+        }
+        for (int s = cnt; s < kMaxSlots; s++) L.st_nbr[(size_t)s * N + n] = n;
+        L.vol[n] = m.cell_volume[i];
+        L.inv_vol[n] = 1.0 / m.cell_volume[i];
+        L.cfl_area[n] = cfl_file[i];
+    }
+
+    // boundary faces
+    L.b_patch.resize(G);
+    L.b_dmag.resize(G);
+    L.b_owner.resize(G);
+    for (int b = 0; b < G; b++) {
+        const int f = NI + b, o = m.owner[f], g = N + b;
+        L.b_patch[b] = bface_patch[b];
+        const double dx = cc[3 * g] - cc[3 * o], dy = cc[3 * g + 1] - cc[3 * o + 1], dz = cc[3 * g + 2] - cc[3 * o + 2];
+        L.b_dmag[b] = std::sqrt(std::pow(dx, 2.0) + std::pow(dy, 2.0) + std::pow(dz, 2.0));
+        L.b_owner[b] = L.inv[o];
+    }
+
+    // block face items and per-cell gather lists
+    const int C = block_cells;
+    L.n_blocks = (N + C - 1) / C;
+    L.blk_item_off.assign(L.n_blocks + 1, 0);
+    L.g_off.assign(N + 1, 0);
+    L.g_ent.clear();
+    L.g_ent.reserve(s_off[N]);
+    std::vector<int32_t> mark(F, -1), slot_of(F, 0);
+    std::vector<int32_t> item_face;
+    item_face.reserve((size_t)F + F / 4);
+    int max_items = 0;
+    for (int blk = 0; blk < L.n_blocks; blk++) {
+        const int c0 = blk * C, c1 = std::min(N, c0 + C);
+        const int first_item = (int)item_face.size();
+        for (int n = c0; n < c1; n++) {
+            const int i = L.perm[n];
+            for (int s = s_off[i]; s < s_off[i + 1]; s++) {
+                const int f = s_face[s];
+                if (f > NI && patches[bface_patch[f - NI]].is_empty) continue;  // Solver.cpp:308
+                if (mark[f] != blk) {
+                    mark[f] = blk;
+                    slot_of[f] = (int)item_face.size() - first_item;
+                    item_face.push_back(f);
+                }
+                L.g_ent.push_back((slot_of[f] << 1) | (m.owner[f] == i ? 1 : 0));
+            }
+            L.g_off[n + 1] = (int32_t)L.g_ent.size();
+        }
+        L.blk_item_off[blk + 1] = (int32_t)item_face.size();
+        max_items = std::max(max_items, (int)item_face.size() - first_item);
+    }
+    L.max_items_per_block = max_items;
+    const size_t E = item_face.size();
+    L.n_items = (int64_t)E;
+    L.it_own.resize(E);
+    L.it_nbr.resize(E);
+    L.it_geo.resize((size_t)kGeoPerItem * E);
+    for (size_t e = 0; e < E; e++) {
+        const int f = item_face[e];
+        const int o = m.owner[f], nb = m.neighbour[f];
+        L.it_own[e] = L.inv[o];
+        L.it_nbr[e] = (f < NI) ? L.inv[nb] : -1 - (f - NI);
+        double g[kGeoPerItem];
+        g[0] = fn[3 * f];
+        g[1] = fn[3 * f + 1];
+        g[2] = fn[3 * f + 2];
+        g[3] = m.face_area[f];
+        g[4] = m.face_dt[f];
+        g[5] = cc[3 * o] - fc[3 * f];
+        g[6] = cc[3 * o + 1] - fc[3 * f + 1];
+        g[7] = cc[3 * o + 2] - fc[3 * f + 2];
+        g[8] = cc[3 * nb] - fc[3 * f];
+        g[9] = cc[3 * nb + 1] - fc[3 * f + 1];
+        g[10] = cc[3 * nb + 2] - fc[3 * f + 2];
+        for (int q = 0; q < kGeoPerItem; q++) L.it_geo[(size_t)q * E + e] = g[q];
+    }
+    return PLBFS_OK;
+}
+
+}  // namespace plbfs

