@@ -1,0 +1,119 @@
+// Solver context shared by the HIP translation units (solver.hip, ibm.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "../host/host_common.hpp"
+#include "../host/layout.hpp"
+#include "lbfs_device.hpp"
+#include "plbfs_hip.h"
+
+#define HIP_TRY(expr)                                                                               \
+    do {                                                                                            \
+        hipError_t e_ = (expr);                                                                     \
+        if (e_ != hipSuccess)                                                                       \
+            return plbfs_fail(PLBFS_ERR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                              __FILE__, __LINE__);                                                  \
+    } while (0)
+
+namespace plbfs {
+constexpr int kBlock = 256;
+}
+
+struct KStat {
+    double ms = 0;
+    int launches = 0;
+};
+
+// Immersed-boundary state of a context (node arrays component-major [3][n]).
+struct IbmState {
+    bool on = false;
+    int n = 0;
+    int kernel = 0;
+    double h = 1, dt = 0, stiffness = 0, node_area = 0, margin = 0;
+    double *x = nullptr, *x0 = nullptr, *xref = nullptr, *vel = nullptr, *force = nullptr, *norm = nullptr;
+    int32_t *np_off = nullptr, *np_cell = nullptr;           // node -> candidate cells
+    int32_t *cp_cell = nullptr, *cp_off = nullptr, *cp_node = nullptr;  // affected cell -> nodes
+    int n_affected = 0;
+    int64_t n_pairs = 0;
+    double *cell_c = nullptr;  // [3][N] centroids, Hilbert order
+};
+
+struct plbfs_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    plbfs::Layout L;
+    plbfs_params params{};
+    std::vector<plbfs_patch> patches;
+    plbfs::DevParams prm{};
+    std::vector<void *> allocs;
+    bool f32 = false;                 // FP32 storage of fields, gradients and face data
+    plbfs::DevMesh<double> dm{};      // FP64 storage (also holds the shared int/FP64 arrays)
+    plbfs::DevMesh<float> dm32{};     // FP32 storage
+    double *stA = nullptr, *stB = nullptr;  // cell records (rho,u,v,w), ping-pong
+    double *t0 = nullptr, *t1 = nullptr;    // conservative RK accumulators (rho, rho u, rho v, rho w)
+    double *grad = nullptr;                 // 12 per cell
+    double *dtl = nullptr;                  // [N]
+    double *fbody = nullptr;                // [3][N] immersed-boundary force density
+    double *partials = nullptr;             // [n_blocks*5]
+    double *bmin = nullptr;                 // [n_blocks]
+    double *hist = nullptr;
+    int hist_cap = 0;
+    plbfs::Ctrl *ctrl = nullptr;
+    size_t lds_bytes = 0;
+    int total_steps = 0;
+    std::vector<double> last_hist;
+    IbmState ibm;
+    // timing
+    bool timing = false;
+    KStat kstat[4];
+    struct EvPair {
+        int which;
+        hipEvent_t a, b;
+    };
+    std::vector<EvPair> pending;
+    // graph of one step
+    hipGraphExec_t graph_exec = nullptr;
+    bool graph_ok = false;
+    bool use_graph = true;
+};
+
+inline int dalloc(plbfs_ctx *c, void **p, size_t bytes) {
+    if (bytes == 0) bytes = 8;
+    HIP_TRY(hipMalloc(p, bytes));
+    c->allocs.push_back(*p);
+    return PLBFS_OK;
+}
+
+template <typename T>
+inline int upload(plbfs_ctx *c, T **dst, const std::vector<T> &src) {
+    int rc = dalloc(c, (void **)dst, src.size() * sizeof(T));
+    if (rc) return rc;
+    if (!src.empty()) HIP_TRY(hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
+    return PLBFS_OK;
+}
+
+// Brackets one launch with events on the context's stream (timing mode only);
+// the pairs are resolved after the run synchronises, so nothing blocks here.
+struct TimedLaunch {
+    plbfs_ctx *c;
+    int which;
+    hipEvent_t a = nullptr, b = nullptr;
+    TimedLaunch(plbfs_ctx *cc, int w) : c(cc), which(w) {
+        if (c->timing) {
+            (void)hipEventCreate(&a);
+            (void)hipEventCreate(&b);
+            (void)hipEventRecord(a, c->stream);
+        }
+    }
+    ~TimedLaunch() {
+        if (c->timing) {
+            (void)hipEventRecord(b, c->stream);
+            c->pending.push_back({which, a, b});
+        }
+    }
+};
+
+// Launches the per-step immersed-boundary kernels (ibm.hip); no-op when off.
+int ibm_launch(plbfs_ctx *c);

@@ -1,0 +1,313 @@
+// Immersed boundary method on unstructured meshes (gfx950).
+//
+// Follows the reference GPU IBM's per-step sequence (src/Solver_gpu.cu:687-718):
+// interpolate node velocities from the step-start state
+// (src/immersed_boundary_method.cu:13-90), move the nodes X = X0 + U dt
+// (:244-295), spring force F = -k dA (X - X_ref) (:161-190) and spread the
+// force onto the cells (:299-375); the cell force enters R every stage
+// (src/Solver_gpu.cu:766-768).  The reference indexes a structured grid and
+// uses a 2-point hat; here every node has a precomputed list of candidate
+// cells (centroids within the kernel support plus a drift margin of the
+// anchor), sorted by cell, so that
+//   interpolation  = one 64-wide wavefront per node, lanes stride over the
+//                    node's pairs, shuffle butterfly reduction (fixed order);
+//   spreading      = one thread per affected cell summing over its pairs in
+//                    node order (pairs regrouped by cell on the host).
+// Weights are normalised per node, W_cn = delta V_c / sum delta V, which is
+// the reference's formula on its uniform unit grid.  No atomics.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <unordered_map>
+#include <vector>
+
+#include "ctx.hpp"
+
+using namespace plbfs;
+
+namespace {
+
+constexpr int kNodesPerBlock = kBlock / 64;
+
+__device__ __forceinline__ double ibm_phi(int kernel, double r) {
+    r = fabs(r);
+    if (kernel == PLBFS_DELTA_HAT2) return r < 1.0 ? 1.0 - r : 0.0;
+    // Peskin, "The immersed boundary method", Acta Numerica 11 (2002), 4-point function
+    if (r <= 1.0) return (3.0 - 2.0 * r + sqrt(1.0 + 4.0 * r - 4.0 * r * r)) / 8.0;
+    if (r <= 2.0) return (5.0 - 2.0 * r - sqrt(-7.0 + 12.0 * r - 4.0 * r * r)) / 8.0;
+    return 0.0;
+}
+
+__device__ __forceinline__ double ibm_delta(int kernel, double ih, const double *__restrict__ cc, int N, int c,
+                                            const double X[3]) {
+    return ibm_phi(kernel, (cc[c] - X[0]) * ih) * ibm_phi(kernel, (cc[(size_t)N + c] - X[1]) * ih) *
+           ibm_phi(kernel, (cc[2 * (size_t)N + c] - X[2]) * ih) * (ih * ih * ih);
+}
+
+__device__ __forceinline__ double wave_allsum(double x) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    return x;
+}
+
+struct IbmArgs {
+    int n, N, kernel;
+    double ih, dt, k_dA, margin;
+    double *x, *x0, *xref, *vel, *force, *norm;
+    const int32_t *np_off, *np_cell;
+    const int32_t *cp_cell, *cp_off, *cp_node;
+    int n_affected;
+    const double *cc, *vol;
+};
+
+template <typename SV>
+__global__ __launch_bounds__(kBlock) void k_ibm_nodes(IbmArgs a, Ctrl *__restrict__ ctrl,
+                                                      const SV *__restrict__ st) {
+    if (ctrl->done) return;
+    const int lane = threadIdx.x & 63;
+    const int k = blockIdx.x * kNodesPerBlock + (threadIdx.x >> 6);
+    if (k >= a.n) return;
+    const int n = a.n;
+    const double X[3] = {a.x[k], a.x[(size_t)n + k], a.x[2 * (size_t)n + k]};
+    const int p0 = a.np_off[k], p1 = a.np_off[k + 1];
+    double S = 0, su = 0, sv = 0, sw = 0;
+    for (int p = p0 + lane; p < p1; p += 64) {
+        const int c = a.np_cell[p];
+        const double d = ibm_delta(a.kernel, a.ih, a.cc, a.N, c, X);
+        if (d != 0.0) {
+            const double wv = d * a.vol[c];
+            const SV s = st[c];
+            S += wv;
+            su += wv * s.y;
+            sv += wv * s.z;
+            sw += wv * s.w;
+        }
+    }
+    S = wave_allsum(S);
+    su = wave_allsum(su);
+    sv = wave_allsum(sv);
+    sw = wave_allsum(sw);
+    const double U[3] = {su / S, sv / S, sw / S};
+    double Xn[3], F[3], drift = 0;
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        const double x0 = a.x0[(size_t)q * n + k], xr = a.xref[(size_t)q * n + k];
+        Xn[q] = x0 + U[q] * a.dt;
+        F[q] = (-a.k_dA * (Xn[q] - xr));
+        drift = fmax(drift, fabs(Xn[q] - xr));
+    }
+    double S2 = 0;
+    for (int p = p0 + lane; p < p1; p += 64) {
+        const int c = a.np_cell[p];
+        const double d = ibm_delta(a.kernel, a.ih, a.cc, a.N, c, Xn);
+        if (d != 0.0) S2 += d * a.vol[c];
+    }
+    S2 = wave_allsum(S2);
+    if (lane == 0) {
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            a.x[(size_t)q * n + k] = Xn[q];
+            a.x0[(size_t)q * n + k] = Xn[q];
+            a.vel[(size_t)q * n + k] = U[q];
+            a.force[(size_t)q * n + k] = F[q];
+        }
+        a.norm[k] = S2;
+        if (!(S > 0.0) || !(S2 > 0.0) || drift > a.margin) {
+            ctrl->status = PLBFS_ERR_IBM;
+            ctrl->done = 1;
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_ibm_spread(IbmArgs a, const Ctrl *__restrict__ ctrl,
+                                                       double *__restrict__ fbody) {
+    if (ctrl->done) return;
+    const int j = blockIdx.x * kBlock + threadIdx.x;
+    if (j >= a.n_affected) return;
+    const int c = a.cp_cell[j];
+    const int n = a.n;
+    double f[3] = {0, 0, 0};
+    for (int p = a.cp_off[j]; p < a.cp_off[j + 1]; p++) {
+        const int k = a.cp_node[p];
+        const double X[3] = {a.x[k], a.x[(size_t)n + k], a.x[2 * (size_t)n + k]};
+        const double d = ibm_delta(a.kernel, a.ih, a.cc, a.N, c, X);
+        if (d == 0.0) continue;
+        const double q = d / a.norm[k];
+        f[0] += a.force[k] * q;
+        f[1] += a.force[(size_t)n + k] * q;
+        f[2] += a.force[2 * (size_t)n + k] * q;
+    }
+    fbody[c] = f[0];
+    fbody[(size_t)a.N + c] = f[1];
+    fbody[2 * (size_t)a.N + c] = f[2];
+}
+
+IbmArgs make_args(plbfs_ctx *c) {
+    const IbmState &s = c->ibm;
+    IbmArgs a;
+    a.n = s.n;
+    a.N = c->L.N;
+    a.kernel = s.kernel;
+    a.ih = 1.0 / s.h;
+    a.dt = s.dt;
+    a.k_dA = s.stiffness * s.node_area;
+    a.margin = s.margin;
+    a.x = s.x;
+    a.x0 = s.x0;
+    a.xref = s.xref;
+    a.vel = s.vel;
+    a.force = s.force;
+    a.norm = s.norm;
+    a.np_off = s.np_off;
+    a.np_cell = s.np_cell;
+    a.cp_cell = s.cp_cell;
+    a.cp_off = s.cp_off;
+    a.cp_node = s.cp_node;
+    a.n_affected = s.n_affected;
+    a.cc = s.cell_c;
+    a.vol = c->dm.vol;
+    return a;
+}
+
+}  // namespace
+
+int ibm_launch(plbfs_ctx *c) {
+    if (!c->ibm.on) return PLBFS_OK;
+    const IbmArgs a = make_args(c);
+    TimedLaunch t(c, PLBFS_KSTAT_IBM);
+    const dim3 grid((a.n + kNodesPerBlock - 1) / kNodesPerBlock);
+    if (c->f32)
+        hipLaunchKernelGGL(k_ibm_nodes<float4>, grid, dim3(kBlock), 0, c->stream, a, c->ctrl, (const float4 *)c->stA);
+    else
+        hipLaunchKernelGGL(k_ibm_nodes<double4>, grid, dim3(kBlock), 0, c->stream, a, c->ctrl,
+                           (const double4 *)c->stA);
+    hipLaunchKernelGGL(k_ibm_spread, dim3((a.n_affected + kBlock - 1) / kBlock), dim3(kBlock), 0, c->stream, a,
+                       c->ctrl, c->fbody);
+    HIP_TRY(hipGetLastError());
+    return PLBFS_OK;
+}
+
+extern "C" int plbfs_set_ibm(plbfs_ctx *c, const plbfs_ibm_nodes *nodes) {
+    if (!c || !nodes) return plbfs_fail(PLBFS_ERR_INPUT, "null argument");
+    const int n = nodes->n_nodes;
+    if (n <= 0 || !nodes->x_ref || !nodes->x || !(nodes->h > 0) || nodes->node_area < 0 ||
+        (nodes->kernel != PLBFS_DELTA_HAT2 && nodes->kernel != PLBFS_DELTA_PESKIN4))
+        return plbfs_fail(PLBFS_ERR_INPUT, "bad immersed-boundary node set");
+    const int N = c->L.N;
+    const double h = nodes->h;
+    const double support = (nodes->kernel == PLBFS_DELTA_HAT2 ? 1.0 : 2.0) * h;
+    const double margin = 0.5 * h;
+    const double reach = support + margin;
+    const double *cc = c->L.centroid.data();
+    for (int k = 0; k < n; k++)
+        for (int q = 0; q < 3; q++)
+            if (std::fabs(nodes->x[3 * k + q] - nodes->x_ref[3 * k + q]) > margin)
+                return plbfs_fail(PLBFS_ERR_IBM, "node %d starts farther than %g from its anchor", k, margin);
+    // bin the cells near the node cloud
+    double lo[3], hi[3];
+    for (int q = 0; q < 3; q++) {
+        lo[q] = hi[q] = nodes->x_ref[q];
+        for (int k = 0; k < n; k++) {
+            lo[q] = std::min(lo[q], nodes->x_ref[3 * k + q]);
+            hi[q] = std::max(hi[q], nodes->x_ref[3 * k + q]);
+        }
+        lo[q] -= reach;
+        hi[q] += reach;
+    }
+    auto key3 = [&](int64_t bx, int64_t by, int64_t bz) { return (bx * 1048576 + by) * 1048576 + bz; };
+    std::unordered_map<int64_t, std::vector<int32_t>> bins;
+    for (int i = 0; i < N; i++) {
+        const double x = cc[i], y = cc[(size_t)N + i], z = cc[2 * (size_t)N + i];
+        if (x < lo[0] || x > hi[0] || y < lo[1] || y > hi[1] || z < lo[2] || z > hi[2]) continue;
+        bins[key3((int64_t)std::floor((x - lo[0]) / reach), (int64_t)std::floor((y - lo[1]) / reach),
+                  (int64_t)std::floor((z - lo[2]) / reach))]
+            .push_back(i);
+    }
+    std::vector<int32_t> np_off(n + 1, 0), np_cell;
+    std::vector<std::pair<int32_t, int32_t>> cell_node;  // (cell, node)
+    for (int k = 0; k < n; k++) {
+        const double *X = nodes->x_ref + 3 * k;
+        const int64_t bx = (int64_t)std::floor((X[0] - lo[0]) / reach), by = (int64_t)std::floor((X[1] - lo[1]) / reach),
+                      bz = (int64_t)std::floor((X[2] - lo[2]) / reach);
+        std::vector<int32_t> cand;
+        for (int64_t dx = -1; dx <= 1; dx++)
+            for (int64_t dy = -1; dy <= 1; dy++)
+                for (int64_t dz = -1; dz <= 1; dz++) {
+                    auto it = bins.find(key3(bx + dx, by + dy, bz + dz));
+                    if (it == bins.end()) continue;
+                    for (int32_t i : it->second) {
+                        if (std::fabs(cc[i] - X[0]) < reach && std::fabs(cc[(size_t)N + i] - X[1]) < reach &&
+                            std::fabs(cc[2 * (size_t)N + i] - X[2]) < reach)
+                            cand.push_back(i);
+                    }
+                }
+        std::sort(cand.begin(), cand.end());
+        if (cand.empty()) return plbfs_fail(PLBFS_ERR_IBM, "node %d has no cell within its kernel support", k);
+        for (int32_t i : cand) {
+            np_cell.push_back(i);
+            cell_node.push_back({i, k});
+        }
+        np_off[k + 1] = (int32_t)np_cell.size();
+    }
+    std::sort(cell_node.begin(), cell_node.end());
+    std::vector<int32_t> cp_cell, cp_off(1, 0), cp_node;
+    for (size_t p = 0; p < cell_node.size(); p++) {
+        if (p == 0 || cell_node[p].first != cell_node[p - 1].first) {
+            if (p > 0) cp_off.push_back((int32_t)p);
+            cp_cell.push_back(cell_node[p].first);
+        }
+        cp_node.push_back(cell_node[p].second);
+    }
+    cp_off.push_back((int32_t)cell_node.size());
+    // device arrays
+    IbmState &s = c->ibm;
+    s = IbmState();
+    s.n = n;
+    s.kernel = nodes->kernel;
+    s.h = h;
+    s.dt = nodes->dt;
+    s.stiffness = nodes->stiffness;
+    s.node_area = nodes->node_area;
+    s.margin = margin;
+    s.n_affected = (int)cp_cell.size();
+    s.n_pairs = (int64_t)np_cell.size();
+    std::vector<double> xr((size_t)3 * n), xx((size_t)3 * n), z((size_t)3 * n, 0.0), zn(n, 0.0);
+    for (int k = 0; k < n; k++)
+        for (int q = 0; q < 3; q++) {
+            xr[(size_t)q * n + k] = nodes->x_ref[3 * k + q];
+            xx[(size_t)q * n + k] = nodes->x[3 * k + q];
+        }
+    int rc;
+    if ((rc = upload(c, &s.xref, xr)) || (rc = upload(c, &s.x, xx)) || (rc = upload(c, &s.x0, xx)) ||
+        (rc = upload(c, &s.vel, z)) || (rc = upload(c, &s.force, z)) || (rc = upload(c, &s.norm, zn)) ||
+        (rc = upload(c, &s.np_off, np_off)) || (rc = upload(c, &s.np_cell, np_cell)) ||
+        (rc = upload(c, &s.cp_cell, cp_cell)) || (rc = upload(c, &s.cp_off, cp_off)) ||
+        (rc = upload(c, &s.cp_node, cp_node)) || (rc = upload(c, &s.cell_c, c->L.centroid)))
+        return rc;
+    HIP_TRY(hipMemset(c->fbody, 0, (size_t)3 * N * sizeof(double)));
+    s.on = true;
+    c->prm.use_body_force = 1;
+    if (c->graph_exec) {
+        (void)hipGraphExecDestroy(c->graph_exec);
+        c->graph_exec = nullptr;
+    }
+    c->graph_ok = false;
+    return PLBFS_OK;
+}
+
+extern "C" int plbfs_get_ibm(plbfs_ctx *c, double *x, double *vel, double *force) {
+    if (!c) return plbfs_fail(PLBFS_ERR_INPUT, "null argument");
+    if (!c->ibm.on) return plbfs_fail(PLBFS_ERR_INPUT, "no immersed boundary set");
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    const int n = c->ibm.n;
+    double *src[3] = {c->ibm.x, c->ibm.vel, c->ibm.force};
+    double *dst[3] = {x, vel, force};
+    std::vector<double> h((size_t)3 * n);
+    for (int a = 0; a < 3; a++) {
+        if (!dst[a]) continue;
+        HIP_TRY(hipMemcpy(h.data(), src[a], h.size() * sizeof(double), hipMemcpyDeviceToHost));
+        for (int k = 0; k < n; k++)
+            for (int q = 0; q < 3; q++) dst[a][3 * k + q] = h[(size_t)q * n + k];
+    }
+    return PLBFS_OK;
+}

@@ -46,9 +46,37 @@ struct IbmDev {
     const double *vol;                      // [N]
 };
 
+// Per-cell records gathered by neighbours are stored array-of-structures so a
+// gather touches one 32 B (state) or 96 B (gradient) run instead of 4 or 12
+// separate cache lines.
+template <typename S>
+struct V4T;
+template <>
+struct V4T<double> {
+    using type = double4;
+};
+template <>
+struct V4T<float> {
+    using type = float4;
+};
+template <typename S>
+using V4 = typename V4T<S>::type;
+
+// Workgroups are dispatched round-robin over the 8 XCDs (blockIdx.x mod 8), and
+// each XCD has its own L2.  Remapping gives XCD x a contiguous range of logical
+// blocks, so Hilbert-adjacent cells (and their shared halo) meet in one L2.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_block(int bid, int nblk, int remap) {
+    if (!remap) return bid;
+    const int q = nblk / kXcds, r = nblk % kXcds;
+    const int x = bid % kXcds, k = bid / kXcds;
+    return x * q + (x < r ? x : r) + k;
+}
+
 template <typename S>
 struct DevMesh {
     int N, G, n_blocks, max_items;
+    int xcd_remap;
     int64_t E;
     const uint8_t *st_cnt;
     const int32_t *st_nbr;
@@ -59,11 +87,13 @@ struct DevMesh {
     const int32_t *g_off, *g_ent;
     const int32_t *b_patch;
     const S *b_dmag;
-    const DevPatch *patches;
+    const DevPatch *patches;  // FP32 storage: rho values shifted by -rho0
+    const S *corr;            // FP32 storage: [3][N] rho0/3 sum(+-n A)/V per cell
 };
 
 struct DevParams {
     double visc, gamma, invgam, visc3_invgam, force, cfl, ceff, ref_rho, ref_mom, tol;
+    double rho0;  // FP32 storage: density offset (rho = rho0 + stored value)
     int time_stepping;
     int use_body_force;
 };
@@ -151,11 +181,21 @@ struct Moments {
     T s0, sx, sy, sz, pxx, pyy, pzz, pxy, pxz, pyz;
 };
 
-template <int EX, int EY, int EZ, typename T>
-__device__ __forceinline__ void add_pop(Moments<T> &M, T wk, T r, T u, T v, T w, T invgam) {
+// Equilibrium of one lattice point folded into the running moments.  With
+// DEV (FP32 storage) the density is a deviation r = rho - rho0 and only
+// g = feq - w rho0 = w (r (1 + X) + rho0 X), X = 3 e.u + (4.5 (e.u)^2 - 1.5 u^2)/gamma,
+// is accumulated, so the O(1) part rho0 w never enters single precision.
+template <int EX, int EY, int EZ, bool DEV, typename T>
+__device__ __forceinline__ void add_pop(Moments<T> &M, T wk, T r, T u, T v, T w, T invgam, T rho0) {
     const T eu = (T)EX * u + (T)EY * v + (T)EZ * w;
     const T usq = u * u + v * v + w * w;
-    const T f = wk * r * ((T)1 + (T)3 * eu + ((T)4.5 * eu * eu - (T)1.5 * usq) * invgam);
+    T f;
+    if constexpr (DEV) {
+        const T X = (T)3 * eu + ((T)4.5 * eu * eu - (T)1.5 * usq) * invgam;
+        f = wk * (r * ((T)1 + X) + rho0 * X);
+    } else {
+        f = wk * r * ((T)1 + (T)3 * eu + ((T)4.5 * eu * eu - (T)1.5 * usq) * invgam);
+    }
     M.s0 += f;
     if (EX) M.sx += (T)EX * f;
     if (EY) M.sy += (T)EY * f;
@@ -171,9 +211,9 @@ __device__ __forceinline__ void add_pop(Moments<T> &M, T wk, T r, T u, T v, T w,
 // Lattice pair +e / -e.  Point +e takes the neighbour side iff e.n < 0 and
 // point -e iff e.n > 0 (the reference's "sum of signed normal components > 0"
 // tests, evaluated in x,y,z order so the sign decision is bit-identical).
-template <int EX, int EY, int EZ, typename T>
+template <int EX, int EY, int EZ, bool DEV, typename T>
 __device__ __forceinline__ void lattice_pair(Moments<T> &M, T wk, const T n[3], T dt, const T pI[4], const T gO[12],
-                                             const T pN[4], const T gN[12], T invgam) {
+                                             const T pN[4], const T gN[12], T invgam, T rho0) {
     T c = (T)0;
     if (EX) c = c + (T)EX * n[0];
     if (EY) c = c + (T)EY * n[1];
@@ -190,33 +230,37 @@ __device__ __forceinline__ void lattice_pair(Moments<T> &M, T wk, const T n[3], 
         a[q] = up ? (pN[q] - pn * dt) : (pI[q] - po * dt);
         b[q] = dn ? (pN[q] + pn * dt) : (pI[q] + po * dt);
     }
-    add_pop<EX, EY, EZ, T>(M, wk, a[0], a[1], a[2], a[3], invgam);
-    add_pop<-EX, -EY, -EZ, T>(M, wk, b[0], b[1], b[2], b[3], invgam);
+    add_pop<EX, EY, EZ, DEV, T>(M, wk, a[0], a[1], a[2], a[3], invgam, rho0);
+    add_pop<-EX, -EY, -EZ, DEV, T>(M, wk, b[0], b[1], b[2], b[3], invgam, rho0);
 }
 
-// Flux through a face of area A and unit normal n (owner -> neighbour).
-template <typename T>
+// Flux through a face of area A and unit normal n (owner -> neighbour).  With
+// DEV the momentum flux omits the constant rho0/3 n A, whose sum over a cell's
+// faces is a per-cell geometric constant added separately.
+template <bool DEV, typename T>
 __device__ __forceinline__ void lattice_flux(const T n[3], T A, T dt, T tau, const T p0[4], const T pI[4],
-                                             const T gO[12], const T pN[4], const T gN[12], T invgam, T F[4]) {
+                                             const T gO[12], const T pN[4], const T gN[12], T invgam, T rho0,
+                                             T F[4]) {
     Moments<T> M = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     const T w0 = (T)(2.0 / 9.0), w1 = (T)(1.0 / 9.0), w7 = (T)(1.0 / 72.0);
-    add_pop<0, 0, 0, T>(M, w0, p0[0], p0[1], p0[2], p0[3], invgam);
-    lattice_pair<1, 0, 0, T>(M, w1, n, dt, pI, gO, pN, gN, invgam);
-    lattice_pair<0, 1, 0, T>(M, w1, n, dt, pI, gO, pN, gN, invgam);
-    lattice_pair<0, 0, 1, T>(M, w1, n, dt, pI, gO, pN, gN, invgam);
-    lattice_pair<1, 1, 1, T>(M, w7, n, dt, pI, gO, pN, gN, invgam);
-    lattice_pair<1, 1, -1, T>(M, w7, n, dt, pI, gO, pN, gN, invgam);
-    lattice_pair<1, -1, 1, T>(M, w7, n, dt, pI, gO, pN, gN, invgam);
-    lattice_pair<-1, 1, 1, T>(M, w7, n, dt, pI, gO, pN, gN, invgam);
-    const T rI = M.s0;
+    add_pop<0, 0, 0, DEV, T>(M, w0, p0[0], p0[1], p0[2], p0[3], invgam, rho0);
+    lattice_pair<1, 0, 0, DEV, T>(M, w1, n, dt, pI, gO, pN, gN, invgam, rho0);
+    lattice_pair<0, 1, 0, DEV, T>(M, w1, n, dt, pI, gO, pN, gN, invgam, rho0);
+    lattice_pair<0, 0, 1, DEV, T>(M, w1, n, dt, pI, gO, pN, gN, invgam, rho0);
+    lattice_pair<1, 1, 1, DEV, T>(M, w7, n, dt, pI, gO, pN, gN, invgam, rho0);
+    lattice_pair<1, 1, -1, DEV, T>(M, w7, n, dt, pI, gO, pN, gN, invgam, rho0);
+    lattice_pair<1, -1, 1, DEV, T>(M, w7, n, dt, pI, gO, pN, gN, invgam, rho0);
+    lattice_pair<-1, 1, 1, DEV, T>(M, w7, n, dt, pI, gO, pN, gN, invgam, rho0);
+    const T rI = DEV ? rho0 + M.s0 : M.s0;
     const T ir = (T)1 / rI;
     const T ux = M.sx * ir, uy = M.sy * ir, uz = M.sz * ir;
     const T om = (T)1 - tau;
-    const T third = (T)(1.0 / 3.0);
+    // sum e e^T feq(I) = rho_I (I/3 + u u^T/gamma); in DEV form rho0 I/3 is dropped
+    const T iso = DEV ? M.s0 * (T)(1.0 / 3.0) : rI * (T)(1.0 / 3.0);
     const T k = om * rI;
-    const T mxx = k * (third + ux * ux * invgam) + tau * M.pxx;
-    const T myy = k * (third + uy * uy * invgam) + tau * M.pyy;
-    const T mzz = k * (third + uz * uz * invgam) + tau * M.pzz;
+    const T mxx = om * iso + k * (ux * ux * invgam) + tau * M.pxx;
+    const T myy = om * iso + k * (uy * uy * invgam) + tau * M.pyy;
+    const T mzz = om * iso + k * (uz * uz * invgam) + tau * M.pzz;
     const T mxy = k * (ux * uy * invgam) + tau * M.pxy;
     const T mxz = k * (ux * uz * invgam) + tau * M.pxz;
     const T myz = k * (uy * uz * invgam) + tau * M.pyz;

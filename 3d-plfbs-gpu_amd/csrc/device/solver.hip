@@ -25,24 +25,12 @@
 #include <string>
 #include <vector>
 
-#include "../host/host_common.hpp"
-#include "../host/layout.hpp"
-#include "lbfs_device.hpp"
-#include "plbfs_hip.h"
+#include "ctx.hpp"
 
 using namespace plbfs;
 
-#define HIP_TRY(expr)                                                                               \
-    do {                                                                                            \
-        hipError_t e_ = (expr);                                                                     \
-        if (e_ != hipSuccess)                                                                       \
-            return plbfs_fail(PLBFS_ERR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
-                              __FILE__, __LINE__);                                                  \
-    } while (0)
-
 namespace {
 
-constexpr int kBlock = 256;
 constexpr int kFinalizeThreads = 256;
 
 // ---------------------------------------------------------------------------
@@ -51,13 +39,13 @@ constexpr int kFinalizeThreads = 256;
 
 template <typename S>
 __global__ __launch_bounds__(kBlock) void k_grad(DevMesh<S> m, const Ctrl *__restrict__ ctrl,
-                                                 const S *__restrict__ st, S *__restrict__ grad) {
+                                                 const V4<S> *__restrict__ st, V4<S> *__restrict__ grad) {
     if (ctrl->done) return;
     const int N = m.N;
-    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const int i = xcd_block(blockIdx.x, gridDim.x, m.xcd_remap) * kBlock + threadIdx.x;
     if (i >= N) return;
-    const S *rho = st, *u = st + N, *v = st + 2 * (size_t)N, *w = st + 3 * (size_t)N;
-    const S p0[4] = {rho[i], u[i], v[i], w[i]};
+    const V4<S> q0 = st[i];
+    const S p0[4] = {q0.x, q0.y, q0.z, q0.w};
     S g[12];
 #pragma unroll
     for (int q = 0; q < 12; q++) g[q] = (S)0;
@@ -68,10 +56,11 @@ __global__ __launch_bounds__(kBlock) void k_grad(DevMesh<S> m, const Ctrl *__res
             const int j = m.st_nbr[(size_t)s * N + i];
             S pj[4];
             if (j >= 0) {
-                pj[0] = rho[j];
-                pj[1] = u[j];
-                pj[2] = v[j];
-                pj[3] = w[j];
+                const V4<S> qj = st[j];
+                pj[0] = qj.x;
+                pj[1] = qj.y;
+                pj[2] = qj.z;
+                pj[3] = qj.w;
             } else {
                 const DevPatch P = m.patches[m.b_patch[-1 - j]];
                 ghost_state<S>(P, p0, pj);
@@ -88,24 +77,31 @@ __global__ __launch_bounds__(kBlock) void k_grad(DevMesh<S> m, const Ctrl *__res
             }
         }
     }
-#pragma unroll
-    for (int q = 0; q < 12; q++) grad[(size_t)q * N + i] = g[q];
+    V4<S> *gi = grad + (size_t)3 * i;
+    gi[0] = V4<S>{g[0], g[1], g[2], g[3]};
+    gi[1] = V4<S>{g[4], g[5], g[6], g[7]};
+    gi[2] = V4<S>{g[8], g[9], g[10], g[11]};
 }
 
 template <typename S>
-__device__ __forceinline__ void load_side(const S *__restrict__ st, const S *__restrict__ grad, int N, int c, S p[4],
-                                          S g[12]) {
-#pragma unroll
-    for (int q = 0; q < 4; q++) p[q] = st[(size_t)q * N + c];
-#pragma unroll
-    for (int q = 0; q < 12; q++) g[q] = grad[(size_t)q * N + c];
+__device__ __forceinline__ void load_side(const V4<S> *__restrict__ st, const V4<S> *__restrict__ grad, int c,
+                                          S p[4], S g[12]) {
+    const V4<S> q = st[c];
+    p[0] = q.x;
+    p[1] = q.y;
+    p[2] = q.z;
+    p[3] = q.w;
+    const V4<S> *gc = grad + (size_t)3 * c;
+    const V4<S> a = gc[0], b = gc[1], d = gc[2];
+    g[0] = a.x; g[1] = a.y; g[2] = a.z; g[3] = a.w;
+    g[4] = b.x; g[5] = b.y; g[6] = b.z; g[7] = b.w;
+    g[8] = d.x; g[9] = d.y; g[10] = d.z; g[11] = d.w;
 }
 
 // Flux of face work item e (src/Solver.cpp:286-347 with populate_lattice_macros :840-1274).
 template <typename S>
 __device__ __forceinline__ void item_flux(const DevMesh<S> &m, const DevParams &prm, int64_t e,
-                                          const S *__restrict__ st, const S *__restrict__ grad, S F[4]) {
-    const int N = m.N;
+                                          const V4<S> *__restrict__ st, const V4<S> *__restrict__ grad, S F[4]) {
     const int64_t E = m.E;
     const int o = m.it_own[e];
     const int nb = m.it_nbr[e];
@@ -115,14 +111,14 @@ __device__ __forceinline__ void item_flux(const DevMesh<S> &m, const DevParams &
     const S n[3] = {geo[0], geo[1], geo[2]};
     const S A = geo[3], dt = geo[4];
     S po[4], go[12];
-    load_side<S>(st, grad, N, o, po, go);
+    load_side<S>(st, grad, o, po, go);
     S pI[4], pN[4], p0[4], gN[12];
     if (nb >= 0 || nb == -1) {
         // interior branch: a real neighbour, or the ghost whose index equals
         // n_cells (the reference's strict "neighbour > n_cells" test, Solver.cpp:861)
         S pn[4];
         if (nb >= 0) {
-            load_side<S>(st, grad, N, nb, pn, gN);
+            load_side<S>(st, grad, nb, pn, gN);
         } else {
             const DevPatch P = m.patches[m.b_patch[0]];
             ghost_state<S>(P, po, pn);
@@ -158,8 +154,23 @@ __device__ __forceinline__ void item_flux(const DevMesh<S> &m, const DevParams &
         for (int q = 0; q < 4; q++) pI[q] = pN[q] = p0[q] = fv[q];
     }
     const S tau = (S)prm.visc3_invgam / dt;
-    lattice_flux<S>(n, A, dt, tau, p0, pI, go, pN, gN, (S)prm.invgam, F);
+    lattice_flux<std::is_same<S, float>::value, S>(n, A, dt, tau, p0, pI, go, pN, gN, (S)prm.invgam, (S)prm.rho0,
+                                                   F);
 }
+
+// Per-cell RK accumulator between stages: the running sum t1 itself in FP64
+// storage; in FP32 storage only the increment sum_k beta_k dt R_k (the
+// conservative state stays in t0 in FP64, so small increments keep their bits).
+template <typename S>
+struct AccT {
+    using type = double4;
+};
+template <>
+struct AccT<float> {
+    using type = float4;
+};
+template <typename S>
+using Acc = typename AccT<S>::type;
 
 __device__ __forceinline__ double local_dt(const DevParams &prm, double rho, double u, double ax, double vol) {
     // Solver.cpp:1715-1744 with the x-slot-only CFL areas (A_y = A_z = 0)
@@ -185,15 +196,17 @@ struct StageArgs {
 
 template <typename S>
 __global__ __launch_bounds__(kBlock) void k_flux_update(DevMesh<S> m, DevParams prm, StageArgs sa,
-                                                        const Ctrl *__restrict__ ctrl, const S *__restrict__ st_in,
-                                                        const S *__restrict__ grad, S *__restrict__ st_out,
-                                                        double *__restrict__ t0, double *__restrict__ t1,
+                                                        const Ctrl *__restrict__ ctrl,
+                                                        const V4<S> *__restrict__ st_in,
+                                                        const V4<S> *__restrict__ grad, V4<S> *__restrict__ st_out,
+                                                        double4 *__restrict__ t0, Acc<S> *__restrict__ t1,
                                                         double *__restrict__ dtl, const double *__restrict__ fbody,
                                                         double *__restrict__ partials) {
+    constexpr bool F32 = std::is_same<S, float>::value;
     extern __shared__ double lds[];
     if (ctrl->done) return;
     const int N = m.N;
-    const int blk = blockIdx.x;
+    const int blk = xcd_block(blockIdx.x, gridDim.x, m.xcd_remap);
     const int e0 = m.blk_item_off[blk], e1 = m.blk_item_off[blk + 1];
     const int nslot = m.max_items;
     S *sF = reinterpret_cast<S *>(lds);
@@ -223,13 +236,20 @@ __global__ __launch_bounds__(kBlock) void k_flux_update(DevMesh<S> m, DevParams 
                 for (int q = 0; q < 4; q++) R[q] = R[q] + ((double)sF[q * nslot + sl] * iv);
             }
         }
+        if (F32) {
+            // rho0/3 sum(+-n A)/V of the cell, omitted from the single-precision face fluxes
+            R[1] += (double)m.corr[i];
+            R[2] += (double)m.corr[(size_t)N + i];
+            R[3] += (double)m.corr[2 * (size_t)N + i];
+        }
         if (prm.use_body_force) {
             R[1] += fbody[i];
             R[2] += fbody[(size_t)N + i];
             R[3] += fbody[2 * (size_t)N + i];
         }
         const double dti = dtl[i];
-        const double a0 = t0[i], a1 = t0[(size_t)N + i], a2 = t0[2 * (size_t)N + i], a3 = t0[3 * (size_t)N + i];
+        const double4 ta = t0[i];
+        const double a0 = ta.x, a1 = ta.y, a2 = ta.z, a3 = ta.w;
         // Solver.cpp:412-416
         double f1 = a0 + R[0] * dti * sa.alpha;
         double f2 = a1 + (R[1] + prm.force) * dti * sa.alpha;
@@ -239,31 +259,46 @@ __global__ __launch_bounds__(kBlock) void k_flux_update(DevMesh<S> m, DevParams 
         f3 = f3 / f1;
         f4 = f4 / f1;
         // Solver.cpp:424-428
-        double b0, b1, b2, b3;
-        if (sa.stage == 0) {
-            b0 = a0; b1 = a1; b2 = a2; b3 = a3;
-        } else {
-            b0 = t1[i]; b1 = t1[(size_t)N + i]; b2 = t1[2 * (size_t)N + i]; b3 = t1[3 * (size_t)N + i];
-        }
         const double db = dti * sa.beta;
-        b0 = b0 + db * R[0];
-        b1 = b1 + db * (R[1] + prm.force);
-        b2 = b2 + db * R[2];
-        b3 = b3 + db * R[3];
+        double b0, b1, b2, b3;
+        if (!F32) {
+            if (sa.stage == 0) {
+                b0 = a0; b1 = a1; b2 = a2; b3 = a3;
+            } else {
+                const Acc<S> tb = t1[i];
+                b0 = tb.x; b1 = tb.y; b2 = tb.z; b3 = tb.w;
+            }
+            b0 = b0 + db * R[0];
+            b1 = b1 + db * (R[1] + prm.force);
+            b2 = b2 + db * R[2];
+            b3 = b3 + db * R[3];
+        } else {
+            // increments only; the conservative state is t0 + increments
+            double d0 = 0, d1 = 0, d2 = 0, d3 = 0;
+            if (sa.stage != 0) {
+                const Acc<S> tb = t1[i];
+                d0 = tb.x; d1 = tb.y; d2 = tb.z; d3 = tb.w;
+            }
+            d0 += db * R[0];
+            d1 += db * (R[1] + prm.force);
+            d2 += db * R[2];
+            d3 += db * R[3];
+            if (sa.stage < 3) {
+                b0 = d0; b1 = d1; b2 = d2; b3 = d3;
+            } else {
+                b0 = a0 + d0; b1 = a1 + d1; b2 = a2 + d2; b3 = a3 + d3;
+            }
+        }
+        const double r0 = F32 ? prm.rho0 : 0.0;  // single-precision density is stored as rho - rho0
         if (sa.stage < 3) {
-            st_out[i] = (S)f1;
-            st_out[(size_t)N + i] = (S)f2;
-            st_out[2 * (size_t)N + i] = (S)f3;
-            st_out[3 * (size_t)N + i] = (S)f4;
-            t1[i] = b0; t1[(size_t)N + i] = b1; t1[2 * (size_t)N + i] = b2; t1[3 * (size_t)N + i] = b3;
+            st_out[i] = V4<S>{(S)(f1 - r0), (S)f2, (S)f3, (S)f4};
+            using E = typename std::conditional<F32, float, double>::type;
+            t1[i] = Acc<S>{(E)b0, (E)b1, (E)b2, (E)b3};
         } else {
             // soln = t1 as primitives (Solver.cpp:431-436); it is the next step's temp
             const double s0 = b0, s1 = b1 / b0, s2 = b2 / b0, s3 = b3 / b0;
-            st_out[i] = (S)s0;
-            st_out[(size_t)N + i] = (S)s1;
-            st_out[2 * (size_t)N + i] = (S)s2;
-            st_out[3 * (size_t)N + i] = (S)s3;
-            t0[i] = b0; t0[(size_t)N + i] = b1; t0[2 * (size_t)N + i] = b2; t0[3 * (size_t)N + i] = b3;
+            st_out[i] = V4<S>{(S)(s0 - r0), (S)s1, (S)s2, (S)s3};
+            t0[i] = double4{b0, b1, b2, b3};
             if (prm.time_stepping == PLBFS_DT_CONSTANT) dtl[i] = prm.cfl;
             else dtl[i] = local_dt(prm, s0, s1, m.cfl_area[i], m.vol[i]);
             sq[0] = R[0] * R[0];
@@ -335,15 +370,17 @@ __global__ __launch_bounds__(kBlock) void k_dt_apply(int N, int nblk, int mode, 
     else dtl[i] = mn * pow(2.0, floor(log2(dtl[i] / mn)));
 }
 
+template <typename S>
 __global__ __launch_bounds__(kBlock) void k_init_dt(int N, DevParams prm, const Ctrl *__restrict__ ctrl,
                                                     const double *__restrict__ cfl_area,
-                                                    const double *__restrict__ vol, const double *__restrict__ st,
+                                                    const double *__restrict__ vol, const V4<S> *__restrict__ st,
                                                     double *__restrict__ dtl) {
     if (ctrl->done) return;
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= N) return;
+    const double r0 = std::is_same<S, float>::value ? prm.rho0 : 0.0;
     if (prm.time_stepping == PLBFS_DT_CONSTANT) dtl[i] = prm.cfl;
-    else dtl[i] = local_dt(prm, st[i], st[(size_t)N + i], cfl_area[i], vol[i]);
+    else dtl[i] = local_dt(prm, r0 + (double)st[i].x, (double)st[i].y, cfl_area[i], vol[i]);
 }
 
 __global__ __launch_bounds__(kFinalizeThreads) void k_finalize(int nblk, double n_cells, double ref_rho,
@@ -394,12 +431,14 @@ __global__ __launch_bounds__(kFinalizeThreads) void k_finalize(int nblk, double 
 
 // One residual evaluation for diagnostics: flux gather only (stage-independent).
 template <typename S>
-__global__ __launch_bounds__(kBlock) void k_residual_only(DevMesh<S> m, DevParams prm, const S *__restrict__ st_in,
-                                                          const S *__restrict__ grad, const double *__restrict__ fbody,
+__global__ __launch_bounds__(kBlock) void k_residual_only(DevMesh<S> m, DevParams prm,
+                                                          const V4<S> *__restrict__ st_in,
+                                                          const V4<S> *__restrict__ grad,
+                                                          const double *__restrict__ fbody,
                                                           double *__restrict__ Rout) {
     extern __shared__ double lds[];
     const int N = m.N;
-    const int blk = blockIdx.x;
+    const int blk = xcd_block(blockIdx.x, gridDim.x, m.xcd_remap);
     const int e0 = m.blk_item_off[blk], e1 = m.blk_item_off[blk + 1];
     const int nslot = m.max_items;
     S *sF = reinterpret_cast<S *>(lds);
@@ -422,6 +461,11 @@ __global__ __launch_bounds__(kBlock) void k_residual_only(DevMesh<S> m, DevParam
             R[q] = R[q] + ((ent & 1) ? -v : v);
         }
     }
+    if (std::is_same<S, float>::value) {
+        R[1] += (double)m.corr[i];
+        R[2] += (double)m.corr[(size_t)N + i];
+        R[3] += (double)m.corr[2 * (size_t)N + i];
+    }
     if (prm.use_body_force) {
         R[1] += fbody[i];
         R[2] += fbody[(size_t)N + i];
@@ -435,63 +479,6 @@ __global__ __launch_bounds__(kBlock) void k_residual_only(DevMesh<S> m, DevParam
 // ---------------------------------------------------------------------------
 // Context
 // ---------------------------------------------------------------------------
-
-struct KStat {
-    double ms = 0;
-    int launches = 0;
-};
-
-struct plbfs_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    Layout L;
-    plbfs_params params{};
-    std::vector<plbfs_patch> patches;
-    DevParams prm{};
-    // device buffers (FP64 storage)
-    std::vector<void *> allocs;
-    DevMesh<double> dm{};
-    double *stA = nullptr, *stB = nullptr;  // [4][N] temp ping-pong
-    double *t0 = nullptr, *t1 = nullptr;    // [4][N]
-    double *grad = nullptr;                 // [12][N]
-    double *dtl = nullptr;                  // [N]
-    double *fbody = nullptr;                // [3][N]
-    double *partials = nullptr;             // [n_blocks*5]
-    double *bmin = nullptr;                 // [n_blocks]
-    double *hist = nullptr;
-    int hist_cap = 0;
-    Ctrl *ctrl = nullptr;
-    size_t lds_bytes = 0;
-    int total_steps = 0;
-    std::vector<double> last_hist;
-    // timing
-    bool timing = false;
-    KStat kstat[4];
-    struct EvPair {
-        int which;
-        hipEvent_t a, b;
-    };
-    std::vector<EvPair> pending;
-    // graph of one step
-    hipGraphExec_t graph_exec = nullptr;
-    bool graph_ok = false;
-    bool use_graph = true;
-};
-
-static int dalloc(plbfs_ctx *c, void **p, size_t bytes) {
-    if (bytes == 0) bytes = 8;
-    HIP_TRY(hipMalloc(p, bytes));
-    c->allocs.push_back(*p);
-    return PLBFS_OK;
-}
-
-template <typename T>
-static int upload(plbfs_ctx *c, T **dst, const std::vector<T> &src) {
-    int rc = dalloc(c, (void **)dst, src.size() * sizeof(T));
-    if (rc) return rc;
-    if (!src.empty()) HIP_TRY(hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
-    return PLBFS_OK;
-}
 
 static int release(plbfs_ctx *c) {
     if (c->graph_exec) {
@@ -524,10 +511,10 @@ extern "C" int plbfs_create(const plbfs_mesh_view *mesh, const plbfs_params *par
         return plbfs_fail(PLBFS_ERR_INPUT, "bad time_stepping or gradient_type");
     if (p.precision != PLBFS_FP64 && p.precision != PLBFS_FP32)
         return plbfs_fail(PLBFS_ERR_INPUT, "bad precision %d", p.precision);
-    if (p.precision == PLBFS_FP32) return plbfs_fail(PLBFS_ERR_INPUT, "FP32 storage is built by plbfs_create32");
     if (plbfs_device_count() <= 0) return plbfs_fail(PLBFS_ERR_NODEVICE, "no HIP device visible");
     auto *c = new plbfs_ctx();
     c->params = p;
+    c->f32 = p.precision == PLBFS_FP32;
     c->patches.assign(patches, patches + n_patches);
     int rc = build_layout(*mesh, p, patches, n_patches, bface_patch, kBlock, c->L);
     if (rc) {
@@ -535,7 +522,8 @@ extern "C" int plbfs_create(const plbfs_mesh_view *mesh, const plbfs_params *par
         return rc;
     }
     const Layout &L = c->L;
-    c->lds_bytes = (size_t)L.max_items_per_block * 4 * sizeof(double);
+    const size_t sbytes = c->f32 ? sizeof(float) : sizeof(double);
+    c->lds_bytes = (size_t)L.max_items_per_block * 4 * sbytes;
     if (c->lds_bytes > 64 * 1024) {
         delete c;
         return plbfs_fail(PLBFS_ERR_INPUT, "a block needs %zu B of LDS for %d face slots (limit 64 KiB)",
@@ -556,6 +544,10 @@ extern "C" int plbfs_create(const plbfs_mesh_view *mesh, const plbfs_params *par
     d.n_blocks = L.n_blocks;
     d.max_items = L.max_items_per_block;
     d.E = L.n_items;
+    // measured on MI355X (4.09M tets): the XCD-contiguous remap is ~5% slower
+    // than plain dispatch order for the flux kernel, so it is opt-in
+    const char *nx = std::getenv("PLBFS_XCD_REMAP");
+    d.xcd_remap = (nx && nx[0] == '1') ? 1 : 0;
     std::vector<DevPatch> dp(n_patches);
     for (int q = 0; q < n_patches; q++) {
         dp[q].rho_type = patches[q].rho_type;
@@ -571,19 +563,41 @@ extern "C" int plbfs_create(const plbfs_mesh_view *mesh, const plbfs_params *par
     if (bpatch.empty()) bpatch.push_back(0);
     std::vector<double> bdmag = L.b_dmag;
     if (bdmag.empty()) bdmag.push_back(1.0);
+    d.corr = nullptr;
     if ((rc = upload(c, (uint8_t **)&d.st_cnt, L.st_cnt)) || (rc = upload(c, (int32_t **)&d.st_nbr, L.st_nbr)) ||
-        (rc = upload(c, (double **)&d.st_w, L.st_w)) || (rc = upload(c, (double **)&d.vol, L.vol)) ||
-        (rc = upload(c, (double **)&d.inv_vol, L.inv_vol)) || (rc = upload(c, (double **)&d.cfl_area, L.cfl_area)) ||
+        (rc = upload(c, (double **)&d.vol, L.vol)) || (rc = upload(c, (double **)&d.inv_vol, L.inv_vol)) ||
+        (rc = upload(c, (double **)&d.cfl_area, L.cfl_area)) ||
         (rc = upload(c, (int32_t **)&d.blk_item_off, L.blk_item_off)) ||
         (rc = upload(c, (int32_t **)&d.it_own, L.it_own)) || (rc = upload(c, (int32_t **)&d.it_nbr, L.it_nbr)) ||
-        (rc = upload(c, (double **)&d.it_geo, L.it_geo)) || (rc = upload(c, (int32_t **)&d.g_off, L.g_off)) ||
-        (rc = upload(c, (int32_t **)&d.g_ent, L.g_ent)) || (rc = upload(c, (int32_t **)&d.b_patch, bpatch)) ||
-        (rc = upload(c, (double **)&d.b_dmag, bdmag)) || (rc = upload(c, (DevPatch **)&d.patches, dp)))
+        (rc = upload(c, (int32_t **)&d.g_off, L.g_off)) || (rc = upload(c, (int32_t **)&d.g_ent, L.g_ent)) ||
+        (rc = upload(c, (int32_t **)&d.b_patch, bpatch)) || (rc = upload(c, (DevPatch **)&d.patches, dp)))
         return fail(rc);
-    const size_t n4 = (size_t)4 * N * sizeof(double);
-    if ((rc = dalloc(c, (void **)&c->stA, n4)) || (rc = dalloc(c, (void **)&c->stB, n4)) ||
-        (rc = dalloc(c, (void **)&c->t0, n4)) || (rc = dalloc(c, (void **)&c->t1, n4)) ||
-        (rc = dalloc(c, (void **)&c->grad, (size_t)12 * N * sizeof(double))) ||
+    if (!c->f32) {
+        if ((rc = upload(c, (double **)&d.st_w, L.st_w)) || (rc = upload(c, (double **)&d.it_geo, L.it_geo)) ||
+            (rc = upload(c, (double **)&d.b_dmag, bdmag)))
+            return fail(rc);
+    } else {
+        // FP32 storage: the same arrays rounded once from the FP64 precompute;
+        // patch densities become deviations from rho0
+        DevMesh<float> &e = c->dm32;
+        e.N = d.N; e.G = d.G; e.n_blocks = d.n_blocks; e.max_items = d.max_items; e.E = d.E;
+        e.xcd_remap = d.xcd_remap;
+        e.st_cnt = d.st_cnt; e.st_nbr = d.st_nbr; e.vol = d.vol; e.inv_vol = d.inv_vol; e.cfl_area = d.cfl_area;
+        e.blk_item_off = d.blk_item_off; e.it_own = d.it_own; e.it_nbr = d.it_nbr; e.g_off = d.g_off;
+        e.g_ent = d.g_ent; e.b_patch = d.b_patch;
+        auto to32 = [](const std::vector<double> &v) { return std::vector<float>(v.begin(), v.end()); };
+        std::vector<DevPatch> dp32 = dp;
+        for (auto &q : dp32) q.rho -= p.ref_rho;
+        if ((rc = upload(c, (float **)&e.st_w, to32(L.st_w))) || (rc = upload(c, (float **)&e.it_geo, to32(L.it_geo))) ||
+            (rc = upload(c, (float **)&e.b_dmag, to32(bdmag))) || (rc = upload(c, (float **)&e.corr, to32(L.corr))) ||
+            (rc = upload(c, (DevPatch **)&e.patches, dp32)))
+            return fail(rc);
+    }
+    const size_t n4 = (size_t)4 * N * sbytes;
+    const size_t nacc = (size_t)4 * N * (c->f32 ? sizeof(float) : sizeof(double));
+    if ((rc = dalloc(c, (void **)&c->stA, n4)) || (rc = dalloc(c, (void **)&c->stB, (size_t)4 * N * sizeof(double))) ||
+        (rc = dalloc(c, (void **)&c->t0, (size_t)4 * N * sizeof(double))) || (rc = dalloc(c, (void **)&c->t1, nacc)) ||
+        (rc = dalloc(c, (void **)&c->grad, (size_t)12 * N * sbytes)) ||
         (rc = dalloc(c, (void **)&c->dtl, (size_t)N * sizeof(double))) ||
         (rc = dalloc(c, (void **)&c->fbody, (size_t)3 * N * sizeof(double))) ||
         (rc = dalloc(c, (void **)&c->partials, (size_t)L.n_blocks * 5 * sizeof(double))) ||
@@ -607,6 +621,7 @@ extern "C" int plbfs_create(const plbfs_mesh_view *mesh, const plbfs_params *par
     prm.cfl = p.cfl;
     prm.ceff = p.max_velocity * std::sqrt(1 - p.gamma + std::pow(p.gamma / std::sqrt(3) / p.max_velocity, 2));
     prm.ref_rho = p.ref_rho;
+    prm.rho0 = p.ref_rho;
     prm.ref_mom = p.ref_rho * p.max_velocity;
     prm.tol = p.tolerance;
     prm.time_stepping = p.time_stepping;
@@ -628,30 +643,41 @@ extern "C" int plbfs_destroy(plbfs_ctx *c) {
 static int to_device_state(plbfs_ctx *c, const double *rho, const double *u, const double *v, const double *w,
                            double *dst) {
     const int N = c->L.N;
-    std::vector<double> h((size_t)4 * N);
+    std::vector<double> h((size_t)4 * N);  // cell records (rho,u,v,w), Hilbert order
     const double *src[4] = {rho, u, v, w};
-    for (int q = 0; q < 4; q++)
-        for (int n = 0; n < N; n++) h[(size_t)q * N + n] = src[q][c->L.perm[n]];
+    for (int n = 0; n < N; n++)
+        for (int q = 0; q < 4; q++) h[(size_t)4 * n + q] = src[q][c->L.perm[n]];
     HIP_TRY(hipMemcpy(dst, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
     return PLBFS_OK;
 }
 
+// Copies nf interleaved values per cell back to file order (one array per value).
 static int from_device_fields(plbfs_ctx *c, const double *src, int nf, double *const *out) {
     const int N = c->L.N;
     std::vector<double> h((size_t)nf * N);
     HIP_TRY(hipMemcpy(h.data(), src, h.size() * sizeof(double), hipMemcpyDeviceToHost));
     for (int q = 0; q < nf; q++)
         if (out[q])
-            for (int n = 0; n < N; n++) out[q][c->L.perm[n]] = h[(size_t)q * N + n];
+            for (int n = 0; n < N; n++) out[q][c->L.perm[n]] = h[(size_t)nf * n + q];
     return PLBFS_OK;
 }
 
 extern "C" int plbfs_set_state(plbfs_ctx *c, const double *rho, const double *u, const double *v, const double *w) {
     if (!c || !rho || !u || !v || !w) return plbfs_fail(PLBFS_ERR_INPUT, "null argument");
-    int rc = to_device_state(c, rho, u, v, w, c->stA);
-    if (rc) return rc;
     // the conservative accumulator starts as a copy of the primitives (Solver.cpp:195)
-    HIP_TRY(hipMemcpy(c->t0, c->stA, (size_t)4 * c->L.N * sizeof(double), hipMemcpyDeviceToDevice));
+    int rc = to_device_state(c, rho, u, v, w, c->t0);
+    if (rc) return rc;
+    const int N = c->L.N;
+    if (!c->f32) {
+        HIP_TRY(hipMemcpy(c->stA, c->t0, (size_t)4 * N * sizeof(double), hipMemcpyDeviceToDevice));
+    } else {
+        std::vector<float> h((size_t)4 * N);
+        const double *src[4] = {rho, u, v, w};
+        for (int n = 0; n < N; n++)
+            for (int q = 0; q < 4; q++)
+                h[(size_t)4 * n + q] = (float)(src[q][c->L.perm[n]] - (q == 0 ? c->prm.rho0 : 0.0));
+        HIP_TRY(hipMemcpy(c->stA, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
+    }
     c->total_steps = 0;
     return PLBFS_OK;
 }
@@ -660,31 +686,26 @@ extern "C" int plbfs_get_state(plbfs_ctx *c, double *rho, double *u, double *v, 
     if (!c) return plbfs_fail(PLBFS_ERR_INPUT, "null argument");
     HIP_TRY(hipStreamSynchronize(c->stream));
     double *outs[4] = {rho, u, v, w};
-    return from_device_fields(c, c->stA, 4, outs);
+    if (!c->f32) return from_device_fields(c, c->stA, 4, outs);
+    // FP32 storage: the FP64 conservative state t0 holds the solution (before
+    // any step it still holds the initial primitives)
+    const int N = c->L.N;
+    std::vector<double> h((size_t)4 * N);
+    HIP_TRY(hipMemcpy(h.data(), c->t0, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int n = 0; n < N; n++) {
+        const double *q = &h[(size_t)4 * n];
+        const int i = c->L.perm[n];
+        const double r = q[0];
+        const bool cons = c->total_steps > 0;
+        if (rho) rho[i] = r;
+        if (u) u[i] = cons ? q[1] / r : q[1];
+        if (v) v[i] = cons ? q[2] / r : q[2];
+        if (w) w[i] = cons ? q[3] / r : q[3];
+    }
+    return PLBFS_OK;
 }
 
 // ---- launches -------------------------------------------------------------
-
-// Brackets one launch with events on the context's stream (timing mode only);
-// the pairs are resolved after the run synchronises, so nothing blocks here.
-struct TimedLaunch {
-    plbfs_ctx *c;
-    int which;
-    hipEvent_t a = nullptr, b = nullptr;
-    TimedLaunch(plbfs_ctx *cc, int w) : c(cc), which(w) {
-        if (c->timing) {
-            hipEventCreate(&a);
-            hipEventCreate(&b);
-            hipEventRecord(a, c->stream);
-        }
-    }
-    ~TimedLaunch() {
-        if (c->timing) {
-            hipEventRecord(b, c->stream);
-            c->pending.push_back({which, a, b});
-        }
-    }
-};
 
 static void resolve_timing(plbfs_ctx *c) {
     for (auto &p : c->pending) {
@@ -712,22 +733,39 @@ static int launch_dt_modes(plbfs_ctx *c) {
     return PLBFS_OK;
 }
 
-static int launch_step(plbfs_ctx *c) {
+template <typename S>
+static void launch_stages(plbfs_ctx *c, const DevMesh<S> &dm) {
     const int nb = c->L.n_blocks;
-    double *in = c->stA, *outb = c->stB;
+    void *in = c->stA, *outb = c->stB;
     for (int k = 0; k < 4; k++) {
         {
             TimedLaunch t(c, PLBFS_KSTAT_GRAD);
-            hipLaunchKernelGGL(k_grad<double>, dim3(nb), dim3(kBlock), 0, c->stream, c->dm, c->ctrl, in, c->grad);
+            hipLaunchKernelGGL(k_grad<S>, dim3(nb), dim3(kBlock), 0, c->stream, dm, c->ctrl, (const V4<S> *)in,
+                               (V4<S> *)c->grad);
         }
         StageArgs sa{kAlpha[k], kBeta[k], k};
         {
             TimedLaunch t(c, PLBFS_KSTAT_FLUX);
-            hipLaunchKernelGGL(k_flux_update<double>, dim3(nb), dim3(kBlock), c->lds_bytes, c->stream, c->dm, c->prm,
-                               sa, c->ctrl, in, c->grad, outb, c->t0, c->t1, c->dtl, c->fbody, c->partials);
+            hipLaunchKernelGGL(k_flux_update<S>, dim3(nb), dim3(kBlock), c->lds_bytes, c->stream, dm, c->prm, sa,
+                               c->ctrl, (const V4<S> *)in, (const V4<S> *)c->grad, (V4<S> *)outb,
+                               (double4 *)c->t0, (Acc<S> *)c->t1, c->dtl, c->fbody, c->partials);
         }
         std::swap(in, outb);
     }
+}
+
+template <typename S>
+static void launch_init_dt(plbfs_ctx *c, const DevMesh<S> &dm) {
+    hipLaunchKernelGGL(k_init_dt<S>, dim3(c->L.n_blocks), dim3(kBlock), 0, c->stream, c->L.N, c->prm, c->ctrl,
+                       dm.cfl_area, dm.vol, (const V4<S> *)c->stA, c->dtl);
+}
+
+static int launch_step(plbfs_ctx *c) {
+    const int nb = c->L.n_blocks;
+    int rc = ibm_launch(c);  // once per step, before the stages (Solver_gpu.cu:687-718)
+    if (rc) return rc;
+    if (c->f32) launch_stages<float>(c, c->dm32);
+    else launch_stages<double>(c, c->dm);
     launch_dt_modes(c);
     {
         TimedLaunch t(c, PLBFS_KSTAT_REDUCE);
@@ -773,8 +811,8 @@ extern "C" int plbfs_run(plbfs_ctx *c, int32_t max_steps, plbfs_result *out) {
     if (rc) return rc;
     Ctrl h{};
     HIP_TRY(hipMemcpyAsync(c->ctrl, &h, sizeof(Ctrl), hipMemcpyHostToDevice, c->stream));
-    hipLaunchKernelGGL(k_init_dt, dim3(c->L.n_blocks), dim3(kBlock), 0, c->stream, c->L.N, c->prm, c->ctrl,
-                       c->dm.cfl_area, c->dm.vol, c->stA, c->dtl);
+    if (c->f32) launch_init_dt<float>(c, c->dm32);
+    else launch_init_dt<double>(c, c->dm);
     launch_dt_modes(c);
     HIP_TRY(hipGetLastError());
     if ((rc = build_graph(c))) return rc;
@@ -784,7 +822,7 @@ extern "C" int plbfs_run(plbfs_ctx *c, int32_t max_steps, plbfs_result *out) {
     HIP_TRY(hipEventCreate(&ev1));
     HIP_TRY(hipEventRecord(ev0, c->stream));
     for (int s = 0; s < max_steps; s++) {
-        if (c->graph_ok) {
+        if (c->graph_ok && !c->timing) {
             HIP_TRY(hipGraphLaunch(c->graph_exec, c->stream));
         } else if ((rc = launch_step(c))) {
             return rc;
@@ -831,17 +869,31 @@ extern "C" int plbfs_eval_residual(plbfs_ctx *c, double *grads, double *R) {
     const int N = c->L.N, nb = c->L.n_blocks;
     Ctrl h{};
     HIP_TRY(hipMemcpyAsync(c->ctrl, &h, sizeof(Ctrl), hipMemcpyHostToDevice, c->stream));
-    hipLaunchKernelGGL(k_grad<double>, dim3(nb), dim3(kBlock), 0, c->stream, c->dm, c->ctrl, c->stA, c->grad);
     double *Rd = c->stB;  // scratch: stB is rewritten before it is read in a step
-    hipLaunchKernelGGL(k_residual_only<double>, dim3(nb), dim3(kBlock), c->lds_bytes, c->stream, c->dm, c->prm,
-                       c->stA, c->grad, c->fbody, Rd);
+    if (!c->f32) {
+        hipLaunchKernelGGL(k_grad<double>, dim3(nb), dim3(kBlock), 0, c->stream, c->dm, c->ctrl,
+                           (const double4 *)c->stA, (double4 *)c->grad);
+        hipLaunchKernelGGL(k_residual_only<double>, dim3(nb), dim3(kBlock), c->lds_bytes, c->stream, c->dm, c->prm,
+                           (const double4 *)c->stA, (const double4 *)c->grad, c->fbody, Rd);
+    } else {
+        hipLaunchKernelGGL(k_grad<float>, dim3(nb), dim3(kBlock), 0, c->stream, c->dm32, c->ctrl,
+                           (const float4 *)c->stA, (float4 *)c->grad);
+        hipLaunchKernelGGL(k_residual_only<float>, dim3(nb), dim3(kBlock), c->lds_bytes, c->stream, c->dm32,
+                           c->prm, (const float4 *)c->stA, (const float4 *)c->grad, c->fbody, Rd);
+    }
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(c->stream));
     if (grads) {
         std::vector<double> h2((size_t)12 * N);
-        HIP_TRY(hipMemcpy(h2.data(), c->grad, h2.size() * sizeof(double), hipMemcpyDeviceToHost));
+        if (!c->f32) {
+            HIP_TRY(hipMemcpy(h2.data(), c->grad, h2.size() * sizeof(double), hipMemcpyDeviceToHost));
+        } else {
+            std::vector<float> h1((size_t)12 * N);
+            HIP_TRY(hipMemcpy(h1.data(), c->grad, h1.size() * sizeof(float), hipMemcpyDeviceToHost));
+            h2.assign(h1.begin(), h1.end());
+        }
         for (int n = 0; n < N; n++)
-            for (int q = 0; q < 12; q++) grads[(size_t)c->L.perm[n] * 12 + q] = h2[(size_t)q * N + n];
+            for (int q = 0; q < 12; q++) grads[(size_t)c->L.perm[n] * 12 + q] = h2[(size_t)12 * n + q];
     }
     if (R) {
         std::vector<double> h4((size_t)4 * N);
@@ -856,8 +908,8 @@ extern "C" int plbfs_get_dt(plbfs_ctx *c, double *dt) {
     if (!c || !dt) return plbfs_fail(PLBFS_ERR_INPUT, "null argument");
     Ctrl h{};
     HIP_TRY(hipMemcpyAsync(c->ctrl, &h, sizeof(Ctrl), hipMemcpyHostToDevice, c->stream));
-    hipLaunchKernelGGL(k_init_dt, dim3(c->L.n_blocks), dim3(kBlock), 0, c->stream, c->L.N, c->prm, c->ctrl,
-                       c->dm.cfl_area, c->dm.vol, c->stA, c->dtl);
+    if (c->f32) launch_init_dt<float>(c, c->dm32);
+    else launch_init_dt<double>(c, c->dm);
     launch_dt_modes(c);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(c->stream));
@@ -895,16 +947,6 @@ extern "C" int plbfs_layout_info(plbfs_ctx *c, int64_t *info) {
     return PLBFS_OK;
 }
 
-// IBM entry points are provided by ibm.hip; placeholders until then.
-extern "C" int plbfs_set_ibm(plbfs_ctx *c, const plbfs_ibm_nodes *nodes) {
-    (void)c;
-    (void)nodes;
-    return plbfs_fail(PLBFS_ERR_INPUT, "immersed boundary not available in this build");
-}
-extern "C" int plbfs_get_ibm(plbfs_ctx *c, double *x, double *vel, double *force) {
-    (void)c; (void)x; (void)vel; (void)force;
-    return plbfs_fail(PLBFS_ERR_INPUT, "immersed boundary not available in this build");
-}
 extern "C" int plbfs_get_body_force(plbfs_ctx *c, double *f) {
     if (!c || !f) return plbfs_fail(PLBFS_ERR_INPUT, "null argument");
     const int N = c->L.N;

@@ -228,8 +228,10 @@ int build_layout(const plbfs_mesh_view &m, const plbfs_params &p, const plbfs_pa
     L.st_w.assign((size_t)kMaxSlots * 3 * N, 0.0);
     L.vol.resize(N);
     L.inv_vol.resize(N);
+    L.centroid.resize((size_t)3 * N);
     for (int n = 0; n < N; n++) {
         const int i = L.perm[n];
+        for (int a = 0; a < 3; a++) L.centroid[(size_t)a * N + n] = cc[3 * i + a];
         const int cnt = s_off[i + 1] - s_off[i];
         L.st_cnt[n] = (uint8_t)cnt;
         const double *q = &L.lsq_inv9[(size_t)i * 9];
@@ -244,6 +246,24 @@ int build_layout(const plbfs_mesh_view &m, const plbfs_params &p, const plbfs_pa
         L.vol[n] = m.cell_volume[i];
         L.inv_vol[n] = 1.0 / m.cell_volume[i];
         L.cfl_area[n] = cfl_file[i];
+    }
+
+    // residual of the uniform rest state rho0 (see layout.hpp)
+    {
+        std::vector<double> cf((size_t)3 * N, 0.0);
+        const double third = p.ref_rho / 3.0;
+        for (int f = 0; f < F; f++) {
+            if (f > NI && patches[bface_patch[f - NI]].is_empty) continue;
+            const int o = m.owner[f];
+            for (int a = 0; a < 3; a++) {
+                const double v = third * fn[3 * f + a] * m.face_area[f];
+                cf[3 * o + a] -= v / m.cell_volume[o];
+                if (f < NI) cf[3 * m.neighbour[f] + a] += v / m.cell_volume[m.neighbour[f]];
+            }
+        }
+        L.corr.resize((size_t)3 * N);
+        for (int n = 0; n < N; n++)
+            for (int a = 0; a < 3; a++) L.corr[(size_t)a * N + n] = cf[3 * L.perm[n] + a];
     }
 
     // boundary faces

@@ -37,6 +37,11 @@ struct Layout {
     std::vector<double> st_w;      // [kMaxSlots*3*N]: folded weights inv(LHS) * RHS_slot, (s*3+c)*N + i
     // per cell
     std::vector<double> vol, inv_vol, cfl_area;  // [N]
+    std::vector<double> centroid;                // [3][N] component-major
+    // rho0/3 * sum over the cell's faces of -+ n A / V (owner -, neighbour +):
+    // the residual of the constant density rho0 at rest, which the FP32
+    // deviation-form fluxes leave out (nonzero only where faces are warped)
+    std::vector<double> corr;  // [3][N]
     // face work items
     std::vector<int32_t> blk_item_off;  // [n_blocks+1]
     std::vector<int32_t> it_own, it_nbr;  // [E] new owner index; neighbour new index or -1-b

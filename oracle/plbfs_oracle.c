@@ -106,6 +106,10 @@ typedef struct {
     double *dtl;                             /* [N] */
     double *force_x, *force_y, *force_z;     /* IBM body force [N], added to R (GPU convention) */
     int use_body_force;
+    /* immersed boundary (Lagrangian nodes) */
+    int ibm_n, ibm_kernel, ibm_on;
+    double ibm_h, ibm_dt, ibm_k, ibm_dA;
+    double *ibm_x, *ibm_x0, *ibm_xref, *ibm_u, *ibm_f; /* [3*n] */
     double res[4];                           /* last residual rho,u,v,w */
     int steps_done;
 } om_ctx;
@@ -380,7 +384,8 @@ void om_destroy(om_ctx *c) {
                     c->Lyx, c->Lyy, c->Lyz, c->Lzx, c->Lzy, c->Lzz, c->Rx, c->Ry, c->Rz, c->cfl_area,
                     c->s_rho, c->s_u, c->s_v, c->s_w, c->t_rho, c->t_u, c->t_v, c->t_w, c->a_rho, c->a_u,
                     c->a_v, c->a_w, c->b_rho, c->b_u, c->b_v, c->b_w, c->r_rho, c->r_u, c->r_v, c->r_w,
-                    c->g, c->dtl, c->force_x, c->force_y, c->force_z};
+                    c->g, c->dtl, c->force_x, c->force_y, c->force_z, c->ibm_x, c->ibm_x0, c->ibm_xref,
+                    c->ibm_u, c->ibm_f};
     for (size_t q = 0; q < sizeof(ptrs) / sizeof(ptrs[0]); q++) free(ptrs[q]);
     free(c);
 }
@@ -818,6 +823,122 @@ void om_get_dt(om_ctx *c, double *dt) {
 }
 
 /* ------------------------------------------------------------------------- */
+/* Immersed boundary: the reference GPU sequence (Solver_gpu.cu:687-718):     */
+/* interpolate node velocities from the step-start state                    */
+/* (immersed_boundary_method.cu:13-90), move nodes X = X0 + U dt             */
+/* (:244-295), spring force F = -k dA (X - X_ref) (:161-190), spread to the   */
+/* cells (:299-375); the cell force is added to R every stage (:766-768).    */
+/* On unstructured meshes the delta weights are normalised per node:        */
+/*   W_cn = delta(x_c - X_n) V_c / sum_c' delta(x_c' - X_n) V_c'             */
+/*   U_n = sum_c W_cn u_c,   f_c = sum_n F_n W_cn / V_c                      */
+/* which is the reference's formula on its uniform unit grid (sum w = 1).   */
+/* Kernels: hat  phi(r) = max(0, 1-|r|)   (the reference's 2-point stencil);  */
+/*          Peskin (2002) 4-point phi on |r| <= 2.                            */
+/* ------------------------------------------------------------------------- */
+
+static double delta_phi(int kernel, double r) {
+    r = fabs(r);
+    if (kernel == 0) return r < 1.0 ? 1.0 - r : 0.0;
+    if (r <= 1.0) return (3.0 - 2.0 * r + sqrt(1.0 + 4.0 * r - 4.0 * r * r)) / 8.0;
+    if (r <= 2.0) return (5.0 - 2.0 * r - sqrt(-7.0 + 12.0 * r - 4.0 * r * r)) / 8.0;
+    return 0.0;
+}
+
+static double delta3(const om_ctx *c, int cell, const double *X) {
+    double ih = 1.0 / c->ibm_h;
+    return delta_phi(c->ibm_kernel, (c->cx[cell] - X[0]) * ih) * delta_phi(c->ibm_kernel, (c->cy[cell] - X[1]) * ih) *
+           delta_phi(c->ibm_kernel, (c->cz[cell] - X[2]) * ih) * (ih * ih * ih);
+}
+
+void om_set_ibm(om_ctx *c, int n, const double *xref, const double *x, double stiffness, double dA, double dt,
+                double h, int kernel) {
+    free(c->ibm_x); free(c->ibm_x0); free(c->ibm_xref); free(c->ibm_u); free(c->ibm_f);
+    c->ibm_n = n;
+    c->ibm_on = n > 0;
+    c->ibm_kernel = kernel;
+    c->ibm_h = h; c->ibm_dt = dt; c->ibm_k = stiffness; c->ibm_dA = dA;
+    c->ibm_x = xcalloc((size_t)3 * n, sizeof(double));
+    c->ibm_x0 = xcalloc((size_t)3 * n, sizeof(double));
+    c->ibm_xref = xcalloc((size_t)3 * n, sizeof(double));
+    c->ibm_u = xcalloc((size_t)3 * n, sizeof(double));
+    c->ibm_f = xcalloc((size_t)3 * n, sizeof(double));
+    memcpy(c->ibm_x, x, sizeof(double) * 3 * n);
+    memcpy(c->ibm_x0, x, sizeof(double) * 3 * n);
+    memcpy(c->ibm_xref, xref, sizeof(double) * 3 * n);
+    c->use_body_force = c->ibm_on;
+    memset(c->force_x, 0, sizeof(double) * c->N);
+    memset(c->force_y, 0, sizeof(double) * c->N);
+    memset(c->force_z, 0, sizeof(double) * c->N);
+}
+
+/* Interpolated velocity at X from the primitive state (brute force over cells). */
+static void ibm_interp(const om_ctx *c, const double *X, const double *u, const double *v, const double *w,
+                       double U[3]) {
+    double S = 0, su = 0, sv = 0, sw = 0;
+    for (int i = 0; i < c->N; i++) {
+        double d = delta3(c, i, X);
+        if (d == 0.0) continue;
+        double wv = d * c->vol[i];
+        S += wv; su += wv * u[i]; sv += wv * v[i]; sw += wv * w[i];
+    }
+    U[0] = su / S; U[1] = sv / S; U[2] = sw / S;
+}
+
+static double ibm_norm(const om_ctx *c, const double *X) {
+    double S = 0;
+    for (int i = 0; i < c->N; i++) {
+        double d = delta3(c, i, X);
+        if (d != 0.0) S += d * c->vol[i];
+    }
+    return S;
+}
+
+/* One IBM update from the given primitive state; fills the cell force arrays. */
+void om_ibm_step(om_ctx *c, const double *u, const double *v, const double *w) {
+    int n = c->ibm_n;
+    memset(c->force_x, 0, sizeof(double) * c->N);
+    memset(c->force_y, 0, sizeof(double) * c->N);
+    memset(c->force_z, 0, sizeof(double) * c->N);
+    for (int k = 0; k < n; k++) {
+        double *X = c->ibm_x + 3 * k, *X0 = c->ibm_x0 + 3 * k, *Xr = c->ibm_xref + 3 * k;
+        double *U = c->ibm_u + 3 * k, *Fn = c->ibm_f + 3 * k;
+        ibm_interp(c, X, u, v, w, U);
+        for (int a = 0; a < 3; a++) {
+            X[a] = X0[a] + U[a] * c->ibm_dt;
+            X0[a] = X[a];
+            Fn[a] = (-c->ibm_k * c->ibm_dA * (X[a] - Xr[a]));
+        }
+    }
+    for (int k = 0; k < n; k++) {
+        double *X = c->ibm_x + 3 * k, *Fn = c->ibm_f + 3 * k;
+        double S = ibm_norm(c, X);
+        for (int i = 0; i < c->N; i++) {
+            double d = delta3(c, i, X);
+            if (d == 0.0) continue;
+            double q = d / S;
+            c->force_x[i] += Fn[0] * q;
+            c->force_y[i] += Fn[1] * q;
+            c->force_z[i] += Fn[2] * q;
+        }
+    }
+}
+
+void om_get_ibm(const om_ctx *c, double *x, double *u, double *f) {
+    size_t n = (size_t)3 * c->ibm_n;
+    if (x) memcpy(x, c->ibm_x, sizeof(double) * n);
+    if (u) memcpy(u, c->ibm_u, sizeof(double) * n);
+    if (f) memcpy(f, c->ibm_f, sizeof(double) * n);
+}
+
+void om_get_body_force(const om_ctx *c, double *f3) {
+    for (int i = 0; i < c->N; i++) {
+        f3[(size_t)3 * i] = c->force_x[i];
+        f3[(size_t)3 * i + 1] = c->force_y[i];
+        f3[(size_t)3 * i + 2] = c->force_z[i];
+    }
+}
+
+/* ------------------------------------------------------------------------- */
 /* Time loop: Solver.cpp:245-560                                             */
 /* ------------------------------------------------------------------------- */
 
@@ -839,6 +960,7 @@ int om_run(om_ctx *c, int nsteps, double *hist, int *steps_done, int *converged)
         memcpy(c->t_rho, c->s_rho, sizeof(double) * T); memcpy(c->t_u, c->s_u, sizeof(double) * T);
         memcpy(c->t_v, c->s_v, sizeof(double) * T); memcpy(c->t_w, c->s_w, sizeof(double) * T);
         get_cfl(c, c->t_rho, c->t_u, c->t_v, c->t_w);
+        if (c->ibm_on) om_ibm_step(c, c->t_u, c->t_v, c->t_w);
         for (int rk = 0; rk < 4; rk++) {
             stage_residual(c);
             for (int i = 0; i < N; i++) {

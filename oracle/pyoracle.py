@@ -196,3 +196,45 @@ class Oracle:
         conv = C.c_int(0)
         rc = lib().om_run(self.h, int(nsteps), _p(hist, C.c_double), C.byref(done), C.byref(conv))
         return dict(rc=rc, steps=done.value, converged=bool(conv.value), hist=hist[:done.value])
+
+
+def _bind_ibm(L):
+    dp = C.POINTER(C.c_double)
+    L.om_set_ibm.argtypes = [C.c_void_p, C.c_int, dp, dp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int]
+    L.om_ibm_step.argtypes = [C.c_void_p, dp, dp, dp]
+    L.om_get_ibm.argtypes = [C.c_void_p, dp, dp, dp]
+    L.om_get_body_force.argtypes = [C.c_void_p, dp]
+
+
+def set_ibm(o, x_ref, x, stiffness, node_area, dt, h, kernel):
+    """Lagrangian nodes for an Oracle (kernel 0 = 2-point hat, 1 = 4-point Peskin)."""
+    L = lib()
+    _bind_ibm(L)
+    xr = np.ascontiguousarray(x_ref, dtype=np.float64).reshape(-1)
+    xx = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
+    L.om_set_ibm(o.h, xr.size // 3, _p(xr, C.c_double), _p(xx, C.c_double), stiffness, node_area, dt, h, kernel)
+    o.n_nodes = xr.size // 3
+
+
+def ibm_step(o, u, v, w):
+    L = lib()
+    _bind_ibm(L)
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (u, v, w)]
+    L.om_ibm_step(o.h, *(_p(a, C.c_double) for a in arrs))
+
+
+def ibm_state(o):
+    L = lib()
+    _bind_ibm(L)
+    n = o.n_nodes
+    x, u, f = (np.zeros(3 * n) for _ in range(3))
+    L.om_get_ibm(o.h, _p(x, C.c_double), _p(u, C.c_double), _p(f, C.c_double))
+    return x.reshape(n, 3), u.reshape(n, 3), f.reshape(n, 3)
+
+
+def body_force(o):
+    L = lib()
+    _bind_ibm(L)
+    f = np.zeros(3 * o.N)
+    L.om_get_body_force(o.h, _p(f, C.c_double))
+    return f.reshape(o.N, 3)

@@ -1,0 +1,233 @@
+"""Host-side checks that need no GPU: library exports, mesh generator, OpenFOAM
+reader round trip, product geometry against the oracle, XML case derivation,
+field and Tecplot output, and clean errors without a device."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from helpers import cavity_mesh, channel_mesh, oracle_for
+from plbfs import _native as nat
+from plbfs import case as pc
+from plbfs import meshgen
+from plbfs.solver import GpuSolver, HostMesh
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    names = set()
+    for h in ("plbfs_hip.h", "plbfs_case.h"):
+        text = open(os.path.join(ROOT, "include", h)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"\b(plbfs_[a-z0-9_]+)\s*\(", text):
+            names.add(m.group(1))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    L = nat.lib()
+    decl = declared_symbols()
+    assert decl == set(nat.EXPORTS)
+    for name in decl:
+        assert hasattr(L, name), name
+
+
+@pytest.mark.parametrize("tets", [False, True])
+def test_meshgen_counts_and_topology(tets):
+    n = (4, 3, 2)
+    pm = meshgen.box_mesh(*n, tets=tets, jitter=0.1, jitter_seed=1, permute_seed=2, patch_types=pc.CHANNEL_PATCHES)
+    H = n[0] * n[1] * n[2]
+    C_ = 6 * H if tets else H
+    bq = 2 * (n[0] * n[1] + n[1] * n[2] + n[0] * n[2])
+    B = 2 * bq if tets else bq
+    fpc = 4 if tets else 6
+    assert pm.n_cells == C_
+    assert pm.n_faces - pm.n_internal == B
+    assert 2 * pm.n_internal + B == fpc * C_
+    assert np.all(pm.owner[: pm.n_internal] < pm.neighbour)
+    # internal faces in upper-triangular order
+    key = pm.owner[: pm.n_internal].astype(np.int64) * pm.n_cells + pm.neighbour
+    assert np.all(np.diff(key) > 0)
+    assert sum(pm.patch_sizes) == B
+
+
+def test_config3_face_counts_formula():
+    # 88^3 x 6 tets: F = (4 * 4088832 + 92928) / 2
+    n = 88
+    cells = 6 * n ** 3
+    bdry = 2 * 2 * 3 * n * n
+    assert cells == 4088832 and bdry == 92928
+    assert (4 * cells + bdry) // 2 == 8224128
+
+
+@pytest.mark.parametrize("tets", [False, True])
+def test_polymesh_roundtrip_through_cpp_reader(tmp_path, tets):
+    pm = cavity_mesh(4, tets=tets, jitter=0.12)
+    prefix = meshgen.write_polymesh(pm, str(tmp_path / "polyMesh"))
+    lines = open(prefix + "owner").read().split("\n")
+    assert lines[12].lstrip().startswith("note") and "nInternalFaces:%d" % pm.n_internal in lines[12]
+    hm = HostMesh.read_openfoam(prefix, visc=0.0005)
+    assert (hm.n_cells, hm.n_faces, hm.n_internal) == (pm.n_cells, pm.n_faces, pm.n_internal)
+    raw = hm.raw()
+    assert np.array_equal(raw["points"], pm.points)  # float32 values survive the text round trip
+    assert np.array_equal(raw["owner"], pm.owner)
+    assert np.array_equal(raw["neighbour"], pm.neighbour)
+    assert np.array_equal(raw["face_off"], pm.face_off)
+    assert np.array_equal(raw["face_verts"], pm.face_verts)
+    assert hm.patch_names == pm.patch_names
+    assert np.array_equal(hm.bface_patch(), pm.bface_patch)
+
+
+def test_reader_errors_are_clean(tmp_path):
+    with pytest.raises(nat.PlbfsError) as e:
+        HostMesh.read_openfoam(str(tmp_path) + "/missing/", visc=0.001)
+    assert e.value.rc == -1
+
+
+@pytest.mark.parametrize("tets,jitter", [(False, 0.15), (True, 0.1)])
+def test_product_geometry_equals_oracle_bitwise(tets, jitter):
+    pm = cavity_mesh(5, tets=tets, jitter=jitter)
+    case = pc.cavity_case()
+    o = oracle_for(pm, case)
+    hm = HostMesh.from_polymesh(pm, case.visc, case.gamma, case.fneq_min)
+    g = hm.geometry()
+    c, f = o.cells(), o.faces()
+    for k in ("cx", "cy", "cz"):
+        assert np.array_equal(g[k], c[k])
+    assert np.array_equal(g["vol"], c["vol"][: pm.n_cells])
+    assert np.array_equal(g["fc"], np.stack([f["fx"], f["fy"], f["fz"]], 1))
+    assert np.array_equal(g["fn"], np.stack([f["ni"], f["nj"], f["nk"]], 1))
+    assert np.array_equal(g["area"], f["area"])
+    assert np.array_equal(g["dt"], f["dt"])
+    assert np.array_equal(g["nbr"], f["nbr"])
+
+
+CASE_XML = """<root>
+<global_variables>
+  <tolerance>1e-6</tolerance><pre_condition_gamma>1</pre_condition_gamma>
+  <simulation_length>20</simulation_length><time_marching_step>0.5</time_marching_step>
+  <reynolds_no>40</reynolds_no><mach_no>0.05</mach_no><simulation_name>chan</simulation_name>
+  <output_directory>{out}</output_directory><max_multi_grid_levels>0</max_multi_grid_levels>
+  <FMG_levels>0</FMG_levels><FMG_tolerance>0</FMG_tolerance><dissipation_kappa_2>0</dissipation_kappa_2>
+  <dissipation_kappa_4>0</dissipation_kappa_4><martinelli_exponent>0</martinelli_exponent>
+  <testcase>1</testcase><mesh_type>3</mesh_type><scale>1</scale><womersley_no>0</womersley_no>
+  <output_step>10</output_step><import_file>{mesh}</import_file><import_format>openfoam</import_format>
+  <gradient_calc>LS</gradient_calc><restart_analysis>false</restart_analysis>
+  <time_stepping>local</time_stepping><preconditioning>{prec}</preconditioning><gpu_solver>1</gpu_solver>
+  <fneq_min>0</fneq_min>
+</global_variables>
+<geometry><x>10</x><y>5</y><z>5</z><dx>1</dx><dy>1</dy><dz>1</dz><streaming_dt>0.5</streaming_dt>
+  <origin_x>0</origin_x><origin_y>0</origin_y><origin_z>0</origin_z><ref_length>4</ref_length></geometry>
+<initial_conditions><average_rho>1</average_rho><pressure_gradient>0</pressure_gradient>
+  <rho><gradient><x>0</x><y>0</y><z>0</z></gradient><origin_magnitude><x>1</x><y>0</y><z>0</z></origin_magnitude></rho>
+  <origin_location><x>0</x><y>0</y><z>0</z></origin_location>
+  <vel><gradient><x>0</x><y>0</y><z>0</z></gradient><origin_magnitude><x>1</x><y>0</y><z>0</z></origin_magnitude></vel>
+  <u>0</u><v>0</v><w>0</w></initial_conditions>
+<boundary_conditions/>
+<uns_boundary_conditions>
+  <bc><name>inlet</name><rho>1</rho><u>1</u><v>0</v><w>0</w><vel_type>dirichlet</vel_type><rho_type>neumann</rho_type></bc>
+  <bc><name>outlet</name><rho>1</rho><u>0</u><v>0</v><w>0</w><vel_type>neumann</vel_type><rho_type>dirichlet</rho_type></bc>
+  <bc><name>farfield</name><rho>1</rho><u>1</u><v>0</v><w>0</w><vel_type>dirichlet</vel_type><rho_type>neumann</rho_type></bc>
+  <periodic_translation><x>0</x><y>0</y><z>0</z></periodic_translation>
+</uns_boundary_conditions>
+</root>
+"""
+
+
+def load_case(path):
+    L = nat.lib()
+    L.plbfs_case_load.argtypes = [C.c_char_p, C.POINTER(C.c_void_p)]
+    L.plbfs_case_params.argtypes = [C.c_void_p, C.POINTER(nat.Params)]
+    L.plbfs_case_patches.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(nat.Patch), C.c_int32]
+    L.plbfs_case_initial_state.argtypes = [C.c_void_p, C.c_void_p] + [C.POINTER(C.c_double)] * 4
+    L.plbfs_case_free.argtypes = [C.c_void_p]
+    h = C.c_void_p()
+    nat.check(L.plbfs_case_load(path.encode(), C.byref(h)))
+    return h
+
+
+@pytest.mark.parametrize("prec", ["false", "true"])
+def test_xml_case_matches_python_derivation(tmp_path, prec):
+    pm = channel_mesh(10, 5, 5, jitter=0.1)
+    prefix = meshgen.write_polymesh(pm, str(tmp_path / "mesh"))
+    xml = tmp_path / "case.xml"
+    xml.write_text(CASE_XML.format(out=str(tmp_path / "out"), mesh=prefix, prec=prec))
+    L = nat.lib()
+    h = load_case(str(xml))
+    p = nat.Params()
+    nat.check(L.plbfs_case_params(h, C.byref(p)))
+    ref = pc.channel_case(re=40.0, length_scale=4.0)
+    ref.preconditioning = prec == "true"
+    rp = ref.params
+    assert p.visc == pytest.approx(rp["visc"], rel=1e-15)
+    assert p.gamma == pytest.approx(rp["gamma"], rel=1e-15)
+    assert p.cfl == 0.5 and p.max_velocity == 0.05 and p.tolerance == 1e-6
+    hm = HostMesh.read_openfoam(prefix, p.visc, p.gamma, p.fneq_min)
+    patches = (nat.Patch * hm.n_patches)()
+    nat.check(L.plbfs_case_patches(h, hm.h, patches, hm.n_patches))
+    exp = ref.patches_for(hm.patch_names)
+    for q in range(hm.n_patches):
+        assert (patches[q].rho_type, patches[q].vel_type) == (exp[q]["rho_type"], exp[q]["vel_type"])
+        assert patches[q].u == pytest.approx(exp[q]["u"], rel=1e-15)
+    T = hm.n_cells + hm.n_ghosts
+    arrs = [np.zeros(T) for _ in range(4)]
+    nat.check(L.plbfs_case_initial_state(h, hm.h, *(nat.dptr(a) for a in arrs)))
+    g = hm.geometry()
+    er = ref.initial_state(g["cx"], g["cy"], g["cz"])
+    for a, b in zip(arrs, er):
+        assert np.allclose(a, b, rtol=1e-15, atol=0)
+    L.plbfs_case_free(h)
+
+
+def test_unknown_patch_token_is_a_clean_error(tmp_path):
+    pm = cavity_mesh(3)
+    prefix = meshgen.write_polymesh(pm, str(tmp_path / "mesh"))
+    xml = tmp_path / "case.xml"
+    xml.write_text(CASE_XML.format(out=str(tmp_path / "out"), mesh=prefix, prec="false"))
+    L = nat.lib()
+    h = load_case(str(xml))
+    hm = HostMesh.read_openfoam(prefix, 0.001)
+    patches = (nat.Patch * hm.n_patches)()
+    assert L.plbfs_case_patches(h, hm.h, patches, hm.n_patches) == -4   # 'wall'/'lid' have no <bc>
+    L.plbfs_case_free(h)
+
+
+def test_fields_roundtrip_full_precision(tmp_path):
+    L = nat.lib()
+    L.plbfs_write_fields.argtypes = [C.c_char_p, C.c_int32] + [C.POINTER(C.c_double)] * 4
+    L.plbfs_read_fields.argtypes = [C.c_char_p, C.c_int32] + [C.POINTER(C.c_double)] * 4
+    rng = np.random.default_rng(0)
+    a = [rng.standard_normal(57) for _ in range(4)]
+    nat.check(L.plbfs_write_fields(str(tmp_path).encode(), 57, *(nat.dptr(x) for x in a)))
+    b = [np.zeros(57) for _ in range(4)]
+    nat.check(L.plbfs_read_fields(str(tmp_path).encode(), 57, *(nat.dptr(x) for x in b)))
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+    assert open(tmp_path / "rho.txt").readline().startswith("0 ,")
+
+
+def test_tecplot_ascii_zone(tmp_path):
+    pm = cavity_mesh(3, tets=True)
+    hm = HostMesh.from_polymesh(pm, 0.001)
+    L = nat.lib()
+    L.plbfs_write_tecplot.argtypes = [C.c_void_p, C.c_char_p] + [C.POINTER(C.c_double)] * 4 + [C.c_double]
+    f = [np.linspace(0, 1, pm.n_cells) for _ in range(4)]
+    path = tmp_path / "sol.dat"
+    nat.check(L.plbfs_write_tecplot(hm.h, str(path).encode(), *(nat.dptr(x) for x in f), 3.0))
+    text = path.read_text()
+    assert "ZONETYPE=FEPOLYHEDRON" in text and "ELEMENTS=%d" % pm.n_cells in text
+    assert "FACES=%d" % pm.n_faces in text
+
+
+def test_create_without_device_is_a_clean_error():
+    if nat.lib().plbfs_device_count() > 0:
+        pytest.skip("a device is visible")
+    pm = cavity_mesh(3)
+    case = pc.cavity_case()
+    hm = HostMesh.from_polymesh(pm, case.visc)
+    with pytest.raises(nat.PlbfsError) as e:
+        GpuSolver(hm, case.params, case.patches_for(pm.patch_names))
+    assert e.value.rc == -6
