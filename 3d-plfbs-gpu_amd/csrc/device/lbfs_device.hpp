@@ -83,6 +83,7 @@ struct DevMesh {
     const S *st_w;
     const double *vol, *inv_vol, *cfl_area;
     const int32_t *blk_item_off, *it_own, *it_nbr;
+    const uint32_t *it_side;  // per item: lattice side bits (see lattice_pair)
     const S *it_geo;
     const int32_t *g_off, *g_ent;
     const int32_t *b_patch;
@@ -208,16 +209,18 @@ __device__ __forceinline__ void add_pop(Moments<T> &M, T wk, T r, T u, T v, T w,
     if (EY * EZ) M.pyz += (T)(EY * EZ) * f;
 }
 
-// Lattice pair +e / -e.  Point +e takes the neighbour side iff e.n < 0 and
-// point -e iff e.n > 0 (the reference's "sum of signed normal components > 0"
-// tests, evaluated in x,y,z order so the sign decision is bit-identical).
-template <int EX, int EY, int EZ, bool DEV, typename T>
-__device__ __forceinline__ void lattice_pair(Moments<T> &M, T wk, const T n[3], T dt, const T pI[4], const T gO[12],
-                                             const T pN[4], const T gN[12], T invgam, T rho0) {
-    T c = (T)0;
-    if (EX) c = c + (T)EX * n[0];
-    if (EY) c = c + (T)EY * n[1];
-    if (EZ) c = c + (T)EZ * n[2];
+// Lattice pair +e / -e (pair index P, order of kLatticePairs).  Point +e takes
+// the neighbour side iff e.n < 0 and point -e iff e.n > 0 (the reference's "sum
+// of signed normal components > 0" tests, src/Solver.cpp:958-1273).  The two
+// decisions per pair are precomputed on the host in FP64 (bits 2P and 2P+1 of
+// `side`): for faces containing a lattice direction, e.n is zero analytically
+// and its sign is rounding noise, so evaluating it in another precision would
+// pick the other side.
+template <int P, int EX, int EY, int EZ, bool DEV, typename T>
+__device__ __forceinline__ void lattice_pair(Moments<T> &M, T wk, unsigned side, T dt, const T pI[4],
+                                             const T gO[12], const T pN[4], const T gN[12], T invgam, T rho0) {
+    const bool up = (side >> (2 * P)) & 1u;      // +e point from the neighbour side
+    const bool dn = (side >> (2 * P + 1)) & 1u;  // -e point from the neighbour side
     T a[4], b[4];
 #pragma unroll
     for (int q = 0; q < 4; q++) {
@@ -225,8 +228,6 @@ __device__ __forceinline__ void lattice_pair(Moments<T> &M, T wk, const T n[3], 
         if (EX) { po += (T)EX * gO[3 * q]; pn += (T)EX * gN[3 * q]; }
         if (EY) { po += (T)EY * gO[3 * q + 1]; pn += (T)EY * gN[3 * q + 1]; }
         if (EZ) { po += (T)EZ * gO[3 * q + 2]; pn += (T)EZ * gN[3 * q + 2]; }
-        const bool up = c < (T)0;   // +e point from the neighbour side
-        const bool dn = c > (T)0;   // -e point from the neighbour side
         a[q] = up ? (pN[q] - pn * dt) : (pI[q] - po * dt);
         b[q] = dn ? (pN[q] + pn * dt) : (pI[q] + po * dt);
     }
@@ -238,19 +239,19 @@ __device__ __forceinline__ void lattice_pair(Moments<T> &M, T wk, const T n[3], 
 // DEV the momentum flux omits the constant rho0/3 n A, whose sum over a cell's
 // faces is a per-cell geometric constant added separately.
 template <bool DEV, typename T>
-__device__ __forceinline__ void lattice_flux(const T n[3], T A, T dt, T tau, const T p0[4], const T pI[4],
-                                             const T gO[12], const T pN[4], const T gN[12], T invgam, T rho0,
-                                             T F[4]) {
+__device__ __forceinline__ void lattice_flux(const T n[3], T A, T dt, T tau, unsigned side, const T p0[4],
+                                             const T pI[4], const T gO[12], const T pN[4], const T gN[12], T invgam,
+                                             T rho0, T F[4]) {
     Moments<T> M = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     const T w0 = (T)(2.0 / 9.0), w1 = (T)(1.0 / 9.0), w7 = (T)(1.0 / 72.0);
     add_pop<0, 0, 0, DEV, T>(M, w0, p0[0], p0[1], p0[2], p0[3], invgam, rho0);
-    lattice_pair<1, 0, 0, DEV, T>(M, w1, n, dt, pI, gO, pN, gN, invgam, rho0);
-    lattice_pair<0, 1, 0, DEV, T>(M, w1, n, dt, pI, gO, pN, gN, invgam, rho0);
-    lattice_pair<0, 0, 1, DEV, T>(M, w1, n, dt, pI, gO, pN, gN, invgam, rho0);
-    lattice_pair<1, 1, 1, DEV, T>(M, w7, n, dt, pI, gO, pN, gN, invgam, rho0);
-    lattice_pair<1, 1, -1, DEV, T>(M, w7, n, dt, pI, gO, pN, gN, invgam, rho0);
-    lattice_pair<1, -1, 1, DEV, T>(M, w7, n, dt, pI, gO, pN, gN, invgam, rho0);
-    lattice_pair<-1, 1, 1, DEV, T>(M, w7, n, dt, pI, gO, pN, gN, invgam, rho0);
+    lattice_pair<0, 1, 0, 0, DEV, T>(M, w1, side, dt, pI, gO, pN, gN, invgam, rho0);
+    lattice_pair<1, 0, 1, 0, DEV, T>(M, w1, side, dt, pI, gO, pN, gN, invgam, rho0);
+    lattice_pair<2, 0, 0, 1, DEV, T>(M, w1, side, dt, pI, gO, pN, gN, invgam, rho0);
+    lattice_pair<3, 1, 1, 1, DEV, T>(M, w7, side, dt, pI, gO, pN, gN, invgam, rho0);
+    lattice_pair<4, 1, 1, -1, DEV, T>(M, w7, side, dt, pI, gO, pN, gN, invgam, rho0);
+    lattice_pair<5, 1, -1, 1, DEV, T>(M, w7, side, dt, pI, gO, pN, gN, invgam, rho0);
+    lattice_pair<6, -1, 1, 1, DEV, T>(M, w7, side, dt, pI, gO, pN, gN, invgam, rho0);
     const T rI = DEV ? rho0 + M.s0 : M.s0;
     const T ir = (T)1 / rI;
     const T ux = M.sx * ir, uy = M.sy * ir, uz = M.sz * ir;

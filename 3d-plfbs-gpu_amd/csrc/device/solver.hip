@@ -154,8 +154,8 @@ __device__ __forceinline__ void item_flux(const DevMesh<S> &m, const DevParams &
         for (int q = 0; q < 4; q++) pI[q] = pN[q] = p0[q] = fv[q];
     }
     const S tau = (S)prm.visc3_invgam / dt;
-    lattice_flux<std::is_same<S, float>::value, S>(n, A, dt, tau, p0, pI, go, pN, gN, (S)prm.invgam, (S)prm.rho0,
-                                                   F);
+    lattice_flux<std::is_same<S, float>::value, S>(n, A, dt, tau, m.it_side[e], p0, pI, go, pN, gN, (S)prm.invgam,
+                                                   (S)prm.rho0, F);
 }
 
 // Per-cell RK accumulator between stages: the running sum t1 itself in FP64
@@ -569,6 +569,7 @@ extern "C" int plbfs_create(const plbfs_mesh_view *mesh, const plbfs_params *par
         (rc = upload(c, (double **)&d.cfl_area, L.cfl_area)) ||
         (rc = upload(c, (int32_t **)&d.blk_item_off, L.blk_item_off)) ||
         (rc = upload(c, (int32_t **)&d.it_own, L.it_own)) || (rc = upload(c, (int32_t **)&d.it_nbr, L.it_nbr)) ||
+        (rc = upload(c, (uint32_t **)&d.it_side, L.it_side)) ||
         (rc = upload(c, (int32_t **)&d.g_off, L.g_off)) || (rc = upload(c, (int32_t **)&d.g_ent, L.g_ent)) ||
         (rc = upload(c, (int32_t **)&d.b_patch, bpatch)) || (rc = upload(c, (DevPatch **)&d.patches, dp)))
         return fail(rc);
@@ -584,6 +585,7 @@ extern "C" int plbfs_create(const plbfs_mesh_view *mesh, const plbfs_params *par
         e.xcd_remap = d.xcd_remap;
         e.st_cnt = d.st_cnt; e.st_nbr = d.st_nbr; e.vol = d.vol; e.inv_vol = d.inv_vol; e.cfl_area = d.cfl_area;
         e.blk_item_off = d.blk_item_off; e.it_own = d.it_own; e.it_nbr = d.it_nbr; e.g_off = d.g_off;
+        e.it_side = d.it_side;
         e.g_ent = d.g_ent; e.b_patch = d.b_patch;
         auto to32 = [](const std::vector<double> &v) { return std::vector<float>(v.begin(), v.end()); };
         std::vector<DevPatch> dp32 = dp;

@@ -314,12 +314,23 @@ int build_layout(const plbfs_mesh_view &m, const plbfs_params &p, const plbfs_pa
     L.n_items = (int64_t)E;
     L.it_own.resize(E);
     L.it_nbr.resize(E);
+    L.it_side.resize(E);
     L.it_geo.resize((size_t)kGeoPerItem * E);
+    static const int kPairs[7][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {1, 1, 1}, {1, 1, -1}, {1, -1, 1}, {-1, 1, 1}};
     for (size_t e = 0; e < E; e++) {
         const int f = item_face[e];
         const int o = m.owner[f], nb = m.neighbour[f];
         L.it_own[e] = L.inv[o];
         L.it_nbr[e] = (f < NI) ? L.inv[nb] : -1 - (f - NI);
+        uint32_t side = 0;
+        for (int pr = 0; pr < 7; pr++) {
+            double c = 0.0;
+            for (int a = 0; a < 3; a++)
+                if (kPairs[pr][a]) c = c + kPairs[pr][a] * fn[3 * f + a];
+            if (c < 0.0) side |= 1u << (2 * pr);
+            if (c > 0.0) side |= 1u << (2 * pr + 1);
+        }
+        L.it_side[e] = side;
         double g[kGeoPerItem];
         g[0] = fn[3 * f];
         g[1] = fn[3 * f + 1];

@@ -45,6 +45,11 @@ struct Layout {
     // face work items
     std::vector<int32_t> blk_item_off;  // [n_blocks+1]
     std::vector<int32_t> it_own, it_nbr;  // [E] new owner index; neighbour new index or -1-b
+    // [E] upwind side of the 14 moving lattice points: for pair p (directions
+    // (1,0,0),(0,1,0),(0,0,1),(1,1,1),(1,1,-1),(1,-1,1),(-1,1,1)) bit 2p is
+    // set when +e takes the neighbour side (e.n < 0), bit 2p+1 when -e does
+    // (e.n > 0); e.n summed in x,y,z order in FP64 as src/Solver.cpp:958-1273
+    std::vector<uint32_t> it_side;
     std::vector<double> it_geo;  // [kGeoPerItem*E], component-major
     // per-cell gather list, entries (slot << 1) | is_owner in file face order
     std::vector<int32_t> g_off;  // [N+1]

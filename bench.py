@@ -86,6 +86,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="tet88")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--precision", choices=("fp64", "fp32"), default="fp64",
+                    help="storage precision of fields, gradients and face data")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a device copy of 1 GiB first (PMC FETCH/WRITE calibration)")
     args = ap.parse_args()
@@ -111,6 +113,7 @@ def main():
           flush=True)
     case.tolerance = 0.0
     case.check_interval = max(args.steps, 1)
+    case.precision = 1 if args.precision == "fp32" else 0
     t0 = time.perf_counter()
     hm = HostMesh.from_polymesh(pm, case.visc, case.gamma, case.fneq_min)
     s = GpuSolver(hm, case.params, case.patches_for(pm.patch_names))
@@ -138,7 +141,8 @@ def main():
     s.run(args.steps)
     kt = {k: s.kernel_stats(i) for k, i in (("grad", 0), ("flux", 1), ("reduce", 3))}
     s.enable_timing(False)
-    g_bytes, f_bytes = algorithmic_bytes(info)
+    P = 4 if args.precision == "fp32" else 8
+    g_bytes, f_bytes = algorithmic_bytes(info, P=P, Gb=P)
     flux_ms = kt["flux"][0] / max(kt["flux"][1], 1)
     grad_ms = kt["grad"][0] / max(kt["grad"][1], 1)
     flux_gbs = f_bytes / (flux_ms * 1e-3) / 1e9
@@ -162,9 +166,9 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.precision == "fp64" else "f32",
         "data": "synthetic mesh (seeded), uniform-inflow initial state",
-        "config": {"workload": desc, "cells": pm.n_cells, "faces": pm.n_faces, "precision": "fp64",
+        "config": {"workload": desc, "cells": pm.n_cells, "faces": pm.n_faces, "precision": args.precision,
                    "ibm": False, "lsq_determinant": "exact" if case.lsq_exact_det else "reference"},
         "roofline": {"bound": "hbm", "achieved": flux_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": flux_gbs / HBM_PEAK_GBS, "traffic": None, "kernel": "k_flux_update",
