@@ -19,6 +19,8 @@
 
 namespace plbfs {
 constexpr int kBlock = 256;
+// default cells per flux block (face slots of one block share its LDS)
+constexpr int kFluxBlockCells = 256;
 }
 
 struct KStat {
@@ -62,6 +64,7 @@ struct plbfs_ctx {
     int hist_cap = 0;
     plbfs::Ctrl *ctrl = nullptr;
     size_t lds_bytes = 0;
+    int cell_grid = 0;  // blocks of kBlock cells for the per-cell kernels
     int total_steps = 0;
     std::vector<double> last_hist;
     IbmState ibm;

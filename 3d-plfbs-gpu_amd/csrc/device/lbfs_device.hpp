@@ -77,6 +77,7 @@ template <typename S>
 struct DevMesh {
     int N, G, n_blocks, max_items;
     int xcd_remap;
+    int block_cells;  // cells per flux block (a multiple of the 256-thread block)
     int64_t E;
     const uint8_t *st_cnt;
     const int32_t *st_nbr;
@@ -84,7 +85,13 @@ struct DevMesh {
     const double *vol, *inv_vol, *cfl_area;
     const int32_t *blk_item_off, *it_own, *it_nbr;
     const uint32_t *it_side;  // per item: lattice side bits (see lattice_pair)
+    // FP64 storage: [5][E] n, A, dt plus it_fc [3][E] face centroid; the
+    // centroid-to-face offsets are formed on the fly from cgeo (same bits as
+    // the host's cc - fc).  FP32 storage: [11][E] n, A, dt, r_owner, r_neighbour.
     const S *it_geo;
+    const double *it_fc;
+    const double4 *cgeo;  // FP64 storage: per cell (cx, cy, cz, 1/V)
+    double g0x, g0y, g0z;  // centroid of the ghost of boundary face 0
     const int32_t *g_off, *g_ent;
     const int32_t *b_patch;
     const S *b_dmag;

@@ -105,9 +105,29 @@ __device__ __forceinline__ void item_flux(const DevMesh<S> &m, const DevParams &
     const int64_t E = m.E;
     const int o = m.it_own[e];
     const int nb = m.it_nbr[e];
-    S geo[kGeoPerItem];
+    S geo[kGeoPerItem];  // n(3), A, dt, r_owner(3), r_neighbour(3)
+    if constexpr (std::is_same<S, double>::value) {
 #pragma unroll
-    for (int q = 0; q < kGeoPerItem; q++) geo[q] = m.it_geo[(size_t)q * E + e];
+        for (int q = 0; q < 5; q++) geo[q] = m.it_geo[(size_t)q * E + e];
+        const double fx = m.it_fc[e], fy = m.it_fc[(size_t)E + e], fz = m.it_fc[2 * (size_t)E + e];
+        const double4 co = m.cgeo[o];
+        geo[5] = co.x - fx;
+        geo[6] = co.y - fy;
+        geo[7] = co.z - fz;
+        if (nb >= 0) {
+            const double4 cn = m.cgeo[nb];
+            geo[8] = cn.x - fx;
+            geo[9] = cn.y - fy;
+            geo[10] = cn.z - fz;
+        } else {
+            geo[8] = m.g0x - fx;
+            geo[9] = m.g0y - fy;
+            geo[10] = m.g0z - fz;
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < kGeoPerItem; q++) geo[q] = m.it_geo[(size_t)q * E + e];
+    }
     const S n[3] = {geo[0], geo[1], geo[2]};
     const S A = geo[3], dt = geo[4];
     S po[4], go[12];
@@ -218,10 +238,10 @@ __global__ __launch_bounds__(kBlock) void k_flux_update(DevMesh<S> m, DevParams 
         for (int q = 0; q < 4; q++) sF[q * nslot + sl] = F[q];
     }
     __syncthreads();
-    const int i = blk * kBlock + threadIdx.x;
     double sq[4] = {0, 0, 0, 0};
     int err = 0;
-    if (i < N) {
+    const int c_end = min(N, (blk + 1) * m.block_cells);
+    for (int i = blk * m.block_cells + threadIdx.x; i < c_end; i += kBlock) {
         const double iv = m.inv_vol[i];
         double R[4] = {0, 0, 0, 0};
         const int k0 = m.g_off[i], k1 = m.g_off[i + 1];
@@ -301,13 +321,13 @@ __global__ __launch_bounds__(kBlock) void k_flux_update(DevMesh<S> m, DevParams 
             t0[i] = double4{b0, b1, b2, b3};
             if (prm.time_stepping == PLBFS_DT_CONSTANT) dtl[i] = prm.cfl;
             else dtl[i] = local_dt(prm, s0, s1, m.cfl_area[i], m.vol[i]);
-            sq[0] = R[0] * R[0];
-            sq[1] = R[1] * R[1];
-            sq[2] = R[2] * R[2];
-            sq[3] = R[3] * R[3];
+            sq[0] += R[0] * R[0];
+            sq[1] += R[1] * R[1];
+            sq[2] += R[2] * R[2];
+            sq[3] += R[3] * R[3];
             // Solver.cpp:448-493 guards on the stage state
             if (isnan(f1) || isnan(f2)) err = 1;
-            else if (f1 / prm.ref_rho > 1000.0) err = 2;
+            else if (f1 / prm.ref_rho > 1000.0 && err == 0) err = 2;
         }
     }
     if (sa.stage == 3) {
@@ -449,29 +469,30 @@ __global__ __launch_bounds__(kBlock) void k_residual_only(DevMesh<S> m, DevParam
         for (int q = 0; q < 4; q++) sF[q * nslot + (e - e0)] = F[q];
     }
     __syncthreads();
-    const int i = blk * kBlock + threadIdx.x;
-    if (i >= N) return;
-    const double iv = m.inv_vol[i];
-    double R[4] = {0, 0, 0, 0};
-    for (int k = m.g_off[i]; k < m.g_off[i + 1]; k++) {
-        const int ent = m.g_ent[k];
-        const int sl = ent >> 1;
-        for (int q = 0; q < 4; q++) {
-            const double v = (double)sF[q * nslot + sl] * iv;
-            R[q] = R[q] + ((ent & 1) ? -v : v);
+    const int c_end = min(N, (blk + 1) * m.block_cells);
+    for (int i = blk * m.block_cells + threadIdx.x; i < c_end; i += kBlock) {
+        const double iv = m.inv_vol[i];
+        double R[4] = {0, 0, 0, 0};
+        for (int k = m.g_off[i]; k < m.g_off[i + 1]; k++) {
+            const int ent = m.g_ent[k];
+            const int sl = ent >> 1;
+            for (int q = 0; q < 4; q++) {
+                const double v = (double)sF[q * nslot + sl] * iv;
+                R[q] = R[q] + ((ent & 1) ? -v : v);
+            }
         }
+        if (std::is_same<S, float>::value) {
+            R[1] += (double)m.corr[i];
+            R[2] += (double)m.corr[(size_t)N + i];
+            R[3] += (double)m.corr[2 * (size_t)N + i];
+        }
+        if (prm.use_body_force) {
+            R[1] += fbody[i];
+            R[2] += fbody[(size_t)N + i];
+            R[3] += fbody[2 * (size_t)N + i];
+        }
+        for (int q = 0; q < 4; q++) Rout[(size_t)q * N + i] = R[q];
     }
-    if (std::is_same<S, float>::value) {
-        R[1] += (double)m.corr[i];
-        R[2] += (double)m.corr[(size_t)N + i];
-        R[3] += (double)m.corr[2 * (size_t)N + i];
-    }
-    if (prm.use_body_force) {
-        R[1] += fbody[i];
-        R[2] += fbody[(size_t)N + i];
-        R[3] += fbody[2 * (size_t)N + i];
-    }
-    for (int q = 0; q < 4; q++) Rout[(size_t)q * N + i] = R[q];
 }
 
 }  // namespace
@@ -516,7 +537,15 @@ extern "C" int plbfs_create(const plbfs_mesh_view *mesh, const plbfs_params *par
     c->params = p;
     c->f32 = p.precision == PLBFS_FP32;
     c->patches.assign(patches, patches + n_patches);
-    int rc = build_layout(*mesh, p, patches, n_patches, bface_patch, kBlock, c->L);
+    // cells per flux block: a multiple of the 256-thread block; larger blocks
+    // duplicate fewer border faces but need more LDS
+    int block_cells = kFluxBlockCells;
+    if (const char *bc = std::getenv("PLBFS_BLOCK_CELLS")) block_cells = std::atoi(bc);
+    if (block_cells < kBlock || block_cells % kBlock || block_cells > 4 * kBlock) {
+        delete c;
+        return plbfs_fail(PLBFS_ERR_INPUT, "PLBFS_BLOCK_CELLS must be a multiple of %d up to %d", kBlock, 4 * kBlock);
+    }
+    int rc = build_layout(*mesh, p, patches, n_patches, bface_patch, block_cells, c->L);
     if (rc) {
         delete c;
         return rc;
@@ -544,6 +573,11 @@ extern "C" int plbfs_create(const plbfs_mesh_view *mesh, const plbfs_params *par
     d.n_blocks = L.n_blocks;
     d.max_items = L.max_items_per_block;
     d.E = L.n_items;
+    d.block_cells = L.block_cells;
+    d.g0x = L.ghost0[0];
+    d.g0y = L.ghost0[1];
+    d.g0z = L.ghost0[2];
+    c->cell_grid = (N + kBlock - 1) / kBlock;
     // measured on MI355X (4.09M tets): the XCD-contiguous remap is ~5% slower
     // than plain dispatch order for the flux kernel, so it is opt-in
     const char *nx = std::getenv("PLBFS_XCD_REMAP");
@@ -574,7 +608,15 @@ extern "C" int plbfs_create(const plbfs_mesh_view *mesh, const plbfs_params *par
         (rc = upload(c, (int32_t **)&d.b_patch, bpatch)) || (rc = upload(c, (DevPatch **)&d.patches, dp)))
         return fail(rc);
     if (!c->f32) {
-        if ((rc = upload(c, (double **)&d.st_w, L.st_w)) || (rc = upload(c, (double **)&d.it_geo, L.it_geo)) ||
+        // n, A, dt only; offsets come from the face and cell centroids
+        std::vector<double> geo5(L.it_geo.begin(), L.it_geo.begin() + (size_t)5 * L.n_items);
+        std::vector<double> cg((size_t)4 * N);
+        for (int n = 0; n < N; n++) {
+            for (int a = 0; a < 3; a++) cg[(size_t)4 * n + a] = L.centroid[(size_t)a * N + n];
+            cg[(size_t)4 * n + 3] = L.inv_vol[n];
+        }
+        if ((rc = upload(c, (double **)&d.st_w, L.st_w)) || (rc = upload(c, (double **)&d.it_geo, geo5)) ||
+            (rc = upload(c, (double **)&d.it_fc, L.it_fc)) || (rc = upload(c, (double **)&d.cgeo, cg)) ||
             (rc = upload(c, (double **)&d.b_dmag, bdmag)))
             return fail(rc);
     } else {
@@ -583,6 +625,10 @@ extern "C" int plbfs_create(const plbfs_mesh_view *mesh, const plbfs_params *par
         DevMesh<float> &e = c->dm32;
         e.N = d.N; e.G = d.G; e.n_blocks = d.n_blocks; e.max_items = d.max_items; e.E = d.E;
         e.xcd_remap = d.xcd_remap;
+        e.block_cells = d.block_cells;
+        e.it_fc = nullptr;
+        e.cgeo = nullptr;
+        e.g0x = d.g0x; e.g0y = d.g0y; e.g0z = d.g0z;
         e.st_cnt = d.st_cnt; e.st_nbr = d.st_nbr; e.vol = d.vol; e.inv_vol = d.inv_vol; e.cfl_area = d.cfl_area;
         e.blk_item_off = d.blk_item_off; e.it_own = d.it_own; e.it_nbr = d.it_nbr; e.g_off = d.g_off;
         e.it_side = d.it_side;
@@ -603,7 +649,7 @@ extern "C" int plbfs_create(const plbfs_mesh_view *mesh, const plbfs_params *par
         (rc = dalloc(c, (void **)&c->dtl, (size_t)N * sizeof(double))) ||
         (rc = dalloc(c, (void **)&c->fbody, (size_t)3 * N * sizeof(double))) ||
         (rc = dalloc(c, (void **)&c->partials, (size_t)L.n_blocks * 5 * sizeof(double))) ||
-        (rc = dalloc(c, (void **)&c->bmin, (size_t)L.n_blocks * sizeof(double))) ||
+        (rc = dalloc(c, (void **)&c->bmin, (size_t)c->cell_grid * sizeof(double))) ||
         (rc = dalloc(c, (void **)&c->ctrl, sizeof(Ctrl))))
         return fail(rc);
     {
@@ -726,7 +772,7 @@ static const double kAlpha[4] = {1.0, 0.5, 0.5, 1.0};
 static const double kBeta[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
 
 static int launch_dt_modes(plbfs_ctx *c) {
-    const int N = c->L.N, nb = c->L.n_blocks;
+    const int N = c->L.N, nb = c->cell_grid;
     if (c->params.time_stepping == PLBFS_DT_MIN || c->params.time_stepping == PLBFS_DT_TALTS) {
         hipLaunchKernelGGL(k_dt_blockmin, dim3(nb), dim3(kBlock), 0, c->stream, N, c->ctrl, c->dtl, c->bmin);
         hipLaunchKernelGGL(k_dt_apply, dim3(nb), dim3(kBlock), 0, c->stream, N, nb, c->params.time_stepping,
@@ -742,7 +788,7 @@ static void launch_stages(plbfs_ctx *c, const DevMesh<S> &dm) {
     for (int k = 0; k < 4; k++) {
         {
             TimedLaunch t(c, PLBFS_KSTAT_GRAD);
-            hipLaunchKernelGGL(k_grad<S>, dim3(nb), dim3(kBlock), 0, c->stream, dm, c->ctrl, (const V4<S> *)in,
+            hipLaunchKernelGGL(k_grad<S>, dim3(c->cell_grid), dim3(kBlock), 0, c->stream, dm, c->ctrl, (const V4<S> *)in,
                                (V4<S> *)c->grad);
         }
         StageArgs sa{kAlpha[k], kBeta[k], k};
@@ -758,7 +804,7 @@ static void launch_stages(plbfs_ctx *c, const DevMesh<S> &dm) {
 
 template <typename S>
 static void launch_init_dt(plbfs_ctx *c, const DevMesh<S> &dm) {
-    hipLaunchKernelGGL(k_init_dt<S>, dim3(c->L.n_blocks), dim3(kBlock), 0, c->stream, c->L.N, c->prm, c->ctrl,
+    hipLaunchKernelGGL(k_init_dt<S>, dim3(c->cell_grid), dim3(kBlock), 0, c->stream, c->L.N, c->prm, c->ctrl,
                        dm.cfl_area, dm.vol, (const V4<S> *)c->stA, c->dtl);
 }
 
@@ -873,12 +919,12 @@ extern "C" int plbfs_eval_residual(plbfs_ctx *c, double *grads, double *R) {
     HIP_TRY(hipMemcpyAsync(c->ctrl, &h, sizeof(Ctrl), hipMemcpyHostToDevice, c->stream));
     double *Rd = c->stB;  // scratch: stB is rewritten before it is read in a step
     if (!c->f32) {
-        hipLaunchKernelGGL(k_grad<double>, dim3(nb), dim3(kBlock), 0, c->stream, c->dm, c->ctrl,
+        hipLaunchKernelGGL(k_grad<double>, dim3(c->cell_grid), dim3(kBlock), 0, c->stream, c->dm, c->ctrl,
                            (const double4 *)c->stA, (double4 *)c->grad);
         hipLaunchKernelGGL(k_residual_only<double>, dim3(nb), dim3(kBlock), c->lds_bytes, c->stream, c->dm, c->prm,
                            (const double4 *)c->stA, (const double4 *)c->grad, c->fbody, Rd);
     } else {
-        hipLaunchKernelGGL(k_grad<float>, dim3(nb), dim3(kBlock), 0, c->stream, c->dm32, c->ctrl,
+        hipLaunchKernelGGL(k_grad<float>, dim3(c->cell_grid), dim3(kBlock), 0, c->stream, c->dm32, c->ctrl,
                            (const float4 *)c->stA, (float4 *)c->grad);
         hipLaunchKernelGGL(k_residual_only<float>, dim3(nb), dim3(kBlock), c->lds_bytes, c->stream, c->dm32,
                            c->prm, (const float4 *)c->stA, (const float4 *)c->grad, c->fbody, Rd);
@@ -946,6 +992,8 @@ extern "C" int plbfs_layout_info(plbfs_ctx *c, int64_t *info) {
     info[5] = L.n_blocks;
     info[6] = L.max_items_per_block;
     info[7] = (int64_t)L.g_ent.size();
+    info[8] = L.block_cells;
+    info[9] = (int64_t)c->lds_bytes;
     return PLBFS_OK;
 }
 

@@ -1,4 +1,4 @@
-// XML case layer and whole-program driver (include/plbfs_case.h).
+// XML case layer and whole-program run (include/plbfs_case.h).
 //
 // Parameter derivation follows src/preprocessor.cpp:27-84 (velocities scaled by
 // mach_no/scale), :187-302 (boundary types by token), :438-515 (globals and
@@ -6,7 +6,8 @@
 // gamma), src/Boundary_Conditions.cpp:388-452 (patch lookup by type token),
 // src/Solution.cpp:94-189 (initial conditions) and src/program.cpp:40-137 (the
 // run sequence).  Lagrangian objects: src/lagrangian_object.cpp:79-113 (rigid
-// cylinder); the rigid sphere with a Fibonacci node lattice is this build's.
+// cylinder); the rigid sphere with a Fibonacci node lattice has no reference
+// counterpart.
 #include <sys/stat.h>
 
 #include <chrono>

@@ -316,6 +316,9 @@ int build_layout(const plbfs_mesh_view &m, const plbfs_params &p, const plbfs_pa
     L.it_nbr.resize(E);
     L.it_side.resize(E);
     L.it_geo.resize((size_t)kGeoPerItem * E);
+    L.it_fc.resize((size_t)3 * E);
+    if (G > 0)
+        for (int a = 0; a < 3; a++) L.ghost0[a] = cc[3 * N + a];
     static const int kPairs[7][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {1, 1, 1}, {1, 1, -1}, {1, -1, 1}, {-1, 1, 1}};
     for (size_t e = 0; e < E; e++) {
         const int f = item_face[e];
@@ -344,6 +347,7 @@ int build_layout(const plbfs_mesh_view &m, const plbfs_params &p, const plbfs_pa
         g[9] = cc[3 * nb + 1] - fc[3 * f + 1];
         g[10] = cc[3 * nb + 2] - fc[3 * f + 2];
         for (int q = 0; q < kGeoPerItem; q++) L.it_geo[(size_t)q * E + e] = g[q];
+        for (int a = 0; a < 3; a++) L.it_fc[(size_t)a * E + e] = fc[3 * f + a];
     }
     return PLBFS_OK;
 }

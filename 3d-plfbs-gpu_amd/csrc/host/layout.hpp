@@ -51,6 +51,8 @@ struct Layout {
     // (e.n > 0); e.n summed in x,y,z order in FP64 as src/Solver.cpp:958-1273
     std::vector<uint32_t> it_side;
     std::vector<double> it_geo;  // [kGeoPerItem*E], component-major
+    std::vector<double> it_fc;   // [3*E] face centroid, component-major
+    double ghost0[3] = {0, 0, 0};  // centroid of the ghost of boundary face 0
     // per-cell gather list, entries (slot << 1) | is_owner in file face order
     std::vector<int32_t> g_off;  // [N+1]
     std::vector<int32_t> g_ent;

@@ -196,7 +196,8 @@ class GpuSolver:
         return ms.value, n.value
 
     def layout_info(self):
-        info = np.zeros(8, dtype=np.int64)
+        info = np.zeros(10, dtype=np.int64)
         nat.check(nat.lib().plbfs_layout_info(self.h, info.ctypes.data_as(C.POINTER(C.c_int64))))
-        keys = ["cells", "faces", "items", "stencil_slots", "ghosts", "blocks", "max_items", "gather"]
+        keys = ["cells", "faces", "items", "stencil_slots", "ghosts", "blocks", "max_items", "gather",
+                "block_cells", "lds_bytes"]
         return dict(zip(keys, (int(x) for x in info)))

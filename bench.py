@@ -4,9 +4,11 @@
 Workload (default): 88^3 hexes split into 6 Kuhn tetrahedra each (4,088,832
 cells, 8,224,128 faces) over a 2:1:1 channel (176 x 88 x 88 lattice units),
 interior vertices jittered by +-10% (seed 4), cell labels permuted (seed 5),
-uniform inflow u = 0.05, Re = 100 on D = 0.15 H.  A "step" is one time step
-(4 RK stages) over every cell; value = cell-updates/s with all data resident
-in HBM.  Prints one JSON line.
+uniform inflow u = 0.05, Re = 100 on D = 0.15 H, with an immersed rigid sphere
+of diameter 0.15 H (2,000 Fibonacci nodes, Peskin 4-point kernel) at
+(0.5 H, 0.5 H, 0.5 H).  A "step" is one time step (IBM update + 4 RK stages)
+over every cell; value = cell-updates/s with all data resident in HBM.
+Prints one JSON line.
 """
 import argparse
 import json
@@ -23,6 +25,7 @@ import numpy as np  # noqa: E402
 from plbfs import _native as nat  # noqa: E402
 from plbfs import case as pc  # noqa: E402
 from plbfs import meshgen  # noqa: E402
+from plbfs import objects  # noqa: E402
 from plbfs.solver import GpuSolver, HostMesh  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
@@ -37,18 +40,22 @@ def workload(name):
         case = pc.channel_case(re=100.0, length_scale=0.15 * n)
         case.lsq_exact_det = 1  # the reference's LSQ determinant diverges on jittered tets
         desc = "88^3x6 Kuhn tets (4,088,832 cells) 2:1:1 channel, jitter 10%, permuted, Re=100"
+        ibm = dict(n_nodes=2000, centre=(0.5 * n, 0.5 * n, 0.5 * n), radius=0.075 * n, stiffness=5.0, h=1.0,
+                   kernel=1)
     elif name == "hex32":
         pm = meshgen.box_mesh(32, 32, 32, patch_types=pc.CAVITY_PATCHES)
         case = pc.cavity_case(re=100.0)
         desc = "lid-driven cavity 32^3 hex (32,768 cells), Re=100"
+        ibm = None
     elif name == "chan96":
         pm = meshgen.box_mesh(96, 48, 48, lengths=(96.0, 48.0, 48.0), jitter=0.15, jitter_seed=2,
                               permute_seed=3, patch_types=pc.CHANNEL_PATCHES)
         case = pc.channel_case(re=40.0, length_scale=12.0)
         desc = "channel 96x48x48 hex (221,184 cells), jitter 15%, permuted, Re=40"
+        ibm = dict(n_nodes=450, centre=(24.0, 24.0, 24.0), radius=6.0, stiffness=5.0, h=1.0, kernel=1)
     else:
         raise SystemExit("unknown workload %s" % name)
-    return pm, case, desc
+    return pm, case, desc, ibm
 
 
 def algorithmic_bytes(info, P=8, Gb=8):
@@ -62,6 +69,19 @@ def algorithmic_bytes(info, P=8, Gb=8):
     grad = N * 16 * P + S * (3 * Gb + 4)
     flux = N * (32 * P + 3 * Gb) + F * (8 + 10 * Gb)
     return grad, flux
+
+
+def measured_traffic(workload, precision, block_cells):
+    """HBM bytes per launch of the flux kernel from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (calibrated against a device copy), or
+    None when no summary matches this workload, precision and block size."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        table = json.load(f)
+    key = "%s/%s/b%d" % (workload, precision, block_cells)
+    return table.get(key)
 
 
 def cpu_baseline(pm, case, st, seconds_hint):
@@ -86,13 +106,14 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="tet88")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-ibm", action="store_true", help="leave the immersed sphere out")
     ap.add_argument("--precision", choices=("fp64", "fp32"), default="fp64",
                     help="storage precision of fields, gradients and face data")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a device copy of 1 GiB first (PMC FETCH/WRITE calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU only (the solver has no multi-GPU path); use --gpus 1",
+        print("bench.py measures one GPU only (the solver has no multi-GPU path); use --gpus 1",
               file=sys.stderr)
         sys.exit(2)
     if nat.lib().plbfs_device_count() <= 0:
@@ -107,7 +128,9 @@ def main():
         print("copy calibration: 1073741824 B read + written", file=sys.stderr, flush=True)
 
     t0 = time.perf_counter()
-    pm, case, desc = workload(args.workload)
+    pm, case, desc, ibm = workload(args.workload)
+    if args.no_ibm:
+        ibm = None
     t_mesh = time.perf_counter() - t0
     print("mesh %s: %d cells %d faces (%.1f s)" % (args.workload, pm.n_cells, pm.n_faces, t_mesh), file=sys.stderr,
           flush=True)
@@ -122,6 +145,10 @@ def main():
     cells = hm.geometry()
     st = case.initial_state(cells["cx"], cells["cy"], cells["cz"])
     s.set_state(*st)
+    if ibm:
+        X, dA = objects.fibonacci_sphere(ibm["n_nodes"], ibm["centre"], ibm["radius"])
+        s.set_ibm(X, X, stiffness=ibm["stiffness"], node_area=dA, dt=case.time_marching_step, h=ibm["h"],
+                  kernel=ibm["kernel"])
     print("setup %.1f s, layout %s" % (t_setup, info), file=sys.stderr, flush=True)
 
     if args.warmup > 0:
@@ -139,7 +166,7 @@ def main():
     # per-kernel durations over the same number of steps with events on the solver stream
     s.enable_timing(True)
     s.run(args.steps)
-    kt = {k: s.kernel_stats(i) for k, i in (("grad", 0), ("flux", 1), ("reduce", 3))}
+    kt = {k: s.kernel_stats(i) for k, i in (("grad", 0), ("flux", 1), ("ibm", 2), ("reduce", 3))}
     s.enable_timing(False)
     P = 4 if args.precision == "fp32" else 8
     g_bytes, f_bytes = algorithmic_bytes(info, P=P, Gb=P)
@@ -147,8 +174,12 @@ def main():
     grad_ms = kt["grad"][0] / max(kt["grad"][1], 1)
     flux_gbs = f_bytes / (flux_ms * 1e-3) / 1e9
     grad_gbs = g_bytes / (grad_ms * 1e-3) / 1e9
-    print("kernels: flux %.3f ms (%.0f GB/s alg), grad %.3f ms (%.0f GB/s alg), reduce %.3f ms" %
-          (flux_ms, flux_gbs, grad_ms, grad_gbs, kt["reduce"][0] / max(kt["reduce"][1], 1)), file=sys.stderr)
+    ibm_ms = kt["ibm"][0] / max(args.steps, 1)
+    print("kernels: flux %.3f ms (%.0f GB/s alg), grad %.3f ms (%.0f GB/s alg), ibm %.3f ms/step, reduce %.3f ms" %
+          (flux_ms, flux_gbs, grad_ms, grad_gbs, ibm_ms, kt["reduce"][0] / max(kt["reduce"][1], 1)),
+          file=sys.stderr)
+    block_cells = int(info["block_cells"])
+    traffic = measured_traffic(args.workload, args.precision, block_cells)
 
     cpu = None
     if not args.no_cpu:
@@ -169,11 +200,16 @@ def main():
         "dtype": "f64" if args.precision == "fp64" else "f32",
         "data": "synthetic mesh (seeded), uniform-inflow initial state",
         "config": {"workload": desc, "cells": pm.n_cells, "faces": pm.n_faces, "precision": args.precision,
-                   "ibm": False, "lsq_determinant": "exact" if case.lsq_exact_det else "reference"},
+                   "ibm": ({"nodes": ibm["n_nodes"], "diameter": 2 * ibm["radius"], "kernel": "peskin4",
+                            "ms_per_step": ibm_ms} if ibm else None),
+                   "block_cells": block_cells,
+                   "lsq_determinant": "exact" if case.lsq_exact_det else "reference"},
         "roofline": {"bound": "hbm", "achieved": flux_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": flux_gbs / HBM_PEAK_GBS, "traffic": None, "kernel": "k_flux_update",
+                     "frac": flux_gbs / HBM_PEAK_GBS, "traffic": traffic, "kernel": "k_flux_update",
                      "kernel_ms": flux_ms, "alg_bytes_per_launch": f_bytes,
-                     "grad_kernel": {"achieved": grad_gbs, "kernel_ms": grad_ms, "alg_bytes_per_launch": g_bytes}},
+                     "grad_kernel": {"achieved": grad_gbs, "kernel_ms": grad_ms, "alg_bytes_per_launch": g_bytes},
+                     "step_alg_bytes_per_cell": 4 * (g_bytes + f_bytes) / pm.n_cells,
+                     "step_achieved": 4 * (g_bytes + f_bytes) / (ms_step * 1e-3) / 1e9},
         "cpu_baseline": cpu,
         "wall_s_per_step_host": wall / args.steps,
     }

@@ -69,7 +69,7 @@ int plbfs_case_patches(const plbfs_case *c, const plbfs_mesh *m, plbfs_patch *ou
 int plbfs_case_initial_state(const plbfs_case *c, const plbfs_mesh *m, double *rho, double *u, double *v, double *w);
 /* Lagrangian objects: number of objects / nodes, and the node set of object k
  * (reference rigid cylinder, src/lagrangian_object.cpp:79-113, or the sphere
- * with Fibonacci nodes that this build adds). */
+ * with Fibonacci nodes, which has no reference counterpart). */
 int plbfs_case_n_objects(const plbfs_case *c);
 int plbfs_case_object_nodes(const plbfs_case *c, int32_t k, double *x_ref3, int32_t max_nodes, int32_t *n_out,
                             double *stiffness, double *node_area, int32_t *kernel);

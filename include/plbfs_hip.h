@@ -9,8 +9,10 @@
  *       post_processing&, std::vector<lagrangian_object>&)
  *   (src/Solver_gpu.h:28-31, called from src/program.cpp:107-110).
  * The reference passes C++ objects; here the same information crosses as plain
- * pointers and sizes.  The C++ adaptor that keeps the reference's call shape
- * lives in 3d-plfbs-gpu_amd/csrc/host/gpu_solver.cpp.
+ * pointers and sizes.  plbfs_program_run (include/plbfs_case.h) is the same
+ * hand-off made from this project's own case reader, in the order of
+ * src/program.cpp:40-137; a reference-side caller fills these structs from its
+ * unstructured_mesh, Boundary_Conditions, Solution and lagrangian_object.
  *
  * Conventions
  *   - All arrays are in the caller's (OpenFOAM file) numbering.  Cells are
@@ -173,9 +175,11 @@ int plbfs_get_dt(plbfs_ctx *ctx, double *dt);
 int plbfs_enable_timing(plbfs_ctx *ctx, int32_t on);
 int plbfs_kernel_stats(plbfs_ctx *ctx, int32_t which, double *total_ms, int32_t *launches);
 
-/* Layout facts for roofline accounting: cells, faces, face work items per
- * stage (faces on block borders are evaluated by both blocks), stencil slots. */
-int plbfs_layout_info(plbfs_ctx *ctx, int64_t *info8);
+/* Layout facts for roofline accounting, written to info[0..9]: cells, faces,
+ * face work items per stage (faces on block borders are evaluated by both
+ * blocks), stencil slots, boundary faces, flux blocks, most items in one
+ * block, gather entries, cells per flux block, LDS bytes per flux block. */
+int plbfs_layout_info(plbfs_ctx *ctx, int64_t *info10);
 
 #define PLBFS_KSTAT_GRAD 0
 #define PLBFS_KSTAT_FLUX 1

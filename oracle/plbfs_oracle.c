@@ -1,10 +1,10 @@
 /*
- * plbfs_oracle.c -- CPU FP64 restatement of the reference solver, used ONLY as a
- * test checker.
+ * plbfs_oracle.c -- CPU FP64 restatement of the reference solver, used ONLY as
+ * the reference result in tests.
  *
  * TEST INFRASTRUCTURE. Only tests/, __graft_entry__.smoke() and bench.py's
- * cpu_baseline leg may load this library, and only as the checker (or as the
- * timed CPU baseline).  The product path (3d-plfbs-gpu_amd/) never links it.
+ * cpu_baseline leg may load this library, and only as the reference result (or as
+ * the timed CPU baseline).  The product path (3d-plfbs-gpu_amd/) never links it.
  *
  * It restates, in plain single-threaded C, the reference's CPU solver
  * (src/Solver.cpp) together with the mesh geometry (src/unstructured_mesh.cpp),
@@ -29,8 +29,28 @@
  * formulation with the 4-point Peskin kernel that the product adds (no
  * reference counterpart).
  *
- * Parity pinning of each function is listed in the test files; functions whose
- * only check is against this restatement are marked "parity unpinned" there.
+ * Pins (tests/test_oracle_pins.py, tests/test_ibm_oracle.py, tests/test_host.py;
+ * the reference ships no tests, fixtures or sample meshes and cannot be built
+ * here, so closed forms and invariants do the pinning):
+ *   build_geometry     cube closed forms, ghost mirror, tet closure, total
+ *                      volume on jittered meshes, single-precision points;
+ *   build_lsq          inv*LHS = I (true determinant) or s*I (reference
+ *                      expression), exact gradients of linear fields on
+ *                      axis-aligned hexes (where the reference expression is exact);
+ *   build_cfl_areas, get_cfl   closed-form dt on uniform cubes (x-slot areas);
+ *   face_flux          Euler flux of a uniform state, zero residual at rest,
+ *                      global conservation against the boundary fluxes;
+ *   om_run (RK)        amplification polynomial 1+Z+7/12Z^2+5/24Z^3+1/24Z^4,
+ *                      body force adds dt*f per step;
+ *   IBM                hat weights equal the reference's structured formula,
+ *                      linear fields reproduced, force conservation, adjointness,
+ *                      node motion X += U dt.
+ * Parity unpinned (restated line by line from the cited source; no vector or
+ * closed form distinguishes them from a plausible alternative): the value of
+ * the reference LSQ determinant expression on non-axis-aligned stencils, the
+ * "neighbour > n_cells" and "face > n_internal" boundary tests, the ghost
+ * gradients of types 2/6/7/8 (incl. the Neumann (1-n) factor without abs), and
+ * the ghost states of types 2/6/7/8.
  */
 #include <math.h>
 #include <stdint.h>

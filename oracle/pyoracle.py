@@ -1,7 +1,7 @@
 """ctypes binding of the CPU oracle (plbfs_oracle.c).
 
 TEST INFRASTRUCTURE: imported only by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py, always as the checker or the timed CPU baseline,
+cpu_baseline leg of bench.py, always as the reference result or the timed CPU baseline,
 never as part of the product path.
 """
 from __future__ import annotations

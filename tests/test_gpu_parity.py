@@ -92,6 +92,25 @@ def test_channel_100_steps(gpu_available, tets):
     assert max(errs) < 1e-10, errs
 
 
+@pytest.mark.parametrize("block_cells", [512, 768, 1024])
+def test_flux_block_sizes(gpu_available, monkeypatch, block_cells):
+    # larger flux blocks change which faces are duplicated on block borders
+    # and how many cells one thread updates, not the arithmetic
+    monkeypatch.setenv("PLBFS_BLOCK_CELLS", str(block_cells))
+    pm = channel_mesh(16, 8, 8, tets=True)
+    case = pc.channel_case(re=40.0, length_scale=6.0)
+    case.lsq_exact_det = 1
+    o, hm, s = make_pair(pm, case)
+    assert s.layout_info()["block_cells"] == block_cells
+    st = perturbed_state(case, o, amp=0.01, seed=4)
+    o.set_state(*st)
+    s.set_state(*st)
+    assert o.run(40)["rc"] == 0
+    assert s.run(40)["rc"] == 0
+    errs = field_errs(s.state(), o.state())
+    assert max(errs) < 1e-10, errs
+
+
 def test_reference_determinant_on_jittered_tets(gpu_available):
     """The reference's LSQ determinant on jittered tets: both sides follow it
     for the few steps before the reference algorithm itself blows up."""

@@ -220,6 +220,31 @@ def test_body_force_adds_dt_f_per_step():
     assert np.allclose(rho, 1.0, atol=1e-15)
 
 
+def test_local_dt_closed_form_with_x_slot_areas():
+    """Uniform cubes of side h: the reference accumulates every face's
+    (|n_x|+|n_y|+|n_z|) A / 2 into the x slot (Solver.cpp:1679-1681), so
+    A_x = 3 h^2 and A_y = A_z = 0, and with u = (U, 0, 0)
+      lambda = (U + c) 3h^2 / gamma + 4 (2 nu / (gamma rho V)) (3h^2)^2,
+      dt = CFL V / lambda.
+    Per-direction areas (h^2 each) would give (U + 3c) h^2 instead."""
+    n = 4
+    h = 1.0 / n
+    pm = meshgen.box_mesh(n, n, n, patch_types=pc.CAVITY_PATCHES)
+    case = pc.cavity_case()
+    o = oracle_for(pm, case)
+    U = 0.05
+    st = [np.full(o.T, 1.0), np.full(o.T, U), np.zeros(o.T), np.zeros(o.T)]
+    o.set_state(*st)
+    dt = o.dt_local()
+    gam, nu, cfl = case.gamma, case.visc, case.time_marching_step
+    c = U * np.sqrt(1 - gam + (gam / np.sqrt(3) / U) ** 2)
+    V, Ax = h ** 3, 3 * h * h
+    lam = (U + c) * Ax / gam + 4 * (2 * nu / (gam * 1.0 * V)) * Ax * Ax
+    assert np.allclose(dt, cfl * V / lam, rtol=1e-12, atol=0)
+    lam_split = (U + 3 * c) * h * h / gam + 4 * (2 * nu / (gam * V)) * 3 * h ** 4
+    assert not np.allclose(dt, cfl * V / lam_split, rtol=1e-3)
+
+
 def test_rk_amplification_polynomial():
     """Single hex cell, Dirichlet no-slip walls: one step from a small perturbation
     of rest equals P(dt J) delta with the reference's stage coefficients."""
