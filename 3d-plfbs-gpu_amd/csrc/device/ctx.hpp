@@ -21,6 +21,7 @@ namespace plbfs {
 constexpr int kBlock = 256;
 // default cells per flux block (face slots of one block share its LDS)
 constexpr int kFluxBlockCells = 256;
+constexpr int kMaxLds = 64 * 1024;  // dynamic LDS per workgroup used by the flux kernels
 }
 
 struct KStat {
@@ -118,5 +119,19 @@ struct TimedLaunch {
     }
 };
 
+// plbfs_create with an explicit flux block size (64..1024 cells, a multiple of
+// 64) and LDS sized for lds_cases cases per block (the batched solver).
+int create_ctx(const plbfs_mesh_view *mesh, const plbfs_params *params, const plbfs_patch *patches,
+               int32_t n_patches, const int32_t *bface_patch, int block_cells, int lds_cases, plbfs_ctx **out);
+
 // Launches the per-step immersed-boundary kernels (ibm.hip); no-op when off.
 int ibm_launch(plbfs_ctx *c);
+
+// Node-major candidate cells of every node (centroid within the kernel support
+// plus a drift margin of the anchor) and the same pairs regrouped by cell.
+struct IbmPairs {
+    double margin = 0;
+    std::vector<int32_t> np_off, np_cell;          // node -> candidate cells (sorted)
+    std::vector<int32_t> cp_cell, cp_off, cp_node;  // affected cell -> nodes (node order)
+};
+int build_ibm_pairs(const plbfs::Layout &L, const plbfs_ibm_nodes *nodes, IbmPairs &P);

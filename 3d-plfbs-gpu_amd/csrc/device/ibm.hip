@@ -22,131 +22,23 @@
 #include <vector>
 
 #include "ctx.hpp"
+#include "ibm_kernels.hpp"
 
 using namespace plbfs;
 
 namespace {
-
-constexpr int kNodesPerBlock = kBlock / 64;
-
-__device__ __forceinline__ double ibm_phi(int kernel, double r) {
-    r = fabs(r);
-    if (kernel == PLBFS_DELTA_HAT2) return r < 1.0 ? 1.0 - r : 0.0;
-    // Peskin, "The immersed boundary method", Acta Numerica 11 (2002), 4-point function
-    if (r <= 1.0) return (3.0 - 2.0 * r + sqrt(1.0 + 4.0 * r - 4.0 * r * r)) / 8.0;
-    if (r <= 2.0) return (5.0 - 2.0 * r - sqrt(-7.0 + 12.0 * r - 4.0 * r * r)) / 8.0;
-    return 0.0;
-}
-
-__device__ __forceinline__ double ibm_delta(int kernel, double ih, const double *__restrict__ cc, int N, int c,
-                                            const double X[3]) {
-    return ibm_phi(kernel, (cc[c] - X[0]) * ih) * ibm_phi(kernel, (cc[(size_t)N + c] - X[1]) * ih) *
-           ibm_phi(kernel, (cc[2 * (size_t)N + c] - X[2]) * ih) * (ih * ih * ih);
-}
-
-__device__ __forceinline__ double wave_allsum(double x) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
-    return x;
-}
-
-struct IbmArgs {
-    int n, N, kernel;
-    double ih, dt, k_dA, margin;
-    double *x, *x0, *xref, *vel, *force, *norm;
-    const int32_t *np_off, *np_cell;
-    const int32_t *cp_cell, *cp_off, *cp_node;
-    int n_affected;
-    const double *cc, *vol;
-};
-
-template <typename SV>
-__global__ __launch_bounds__(kBlock) void k_ibm_nodes(IbmArgs a, Ctrl *__restrict__ ctrl,
-                                                      const SV *__restrict__ st) {
-    if (ctrl->done) return;
-    const int lane = threadIdx.x & 63;
-    const int k = blockIdx.x * kNodesPerBlock + (threadIdx.x >> 6);
-    if (k >= a.n) return;
-    const int n = a.n;
-    const double X[3] = {a.x[k], a.x[(size_t)n + k], a.x[2 * (size_t)n + k]};
-    const int p0 = a.np_off[k], p1 = a.np_off[k + 1];
-    double S = 0, su = 0, sv = 0, sw = 0;
-    for (int p = p0 + lane; p < p1; p += 64) {
-        const int c = a.np_cell[p];
-        const double d = ibm_delta(a.kernel, a.ih, a.cc, a.N, c, X);
-        if (d != 0.0) {
-            const double wv = d * a.vol[c];
-            const SV s = st[c];
-            S += wv;
-            su += wv * s.y;
-            sv += wv * s.z;
-            sw += wv * s.w;
-        }
-    }
-    S = wave_allsum(S);
-    su = wave_allsum(su);
-    sv = wave_allsum(sv);
-    sw = wave_allsum(sw);
-    const double U[3] = {su / S, sv / S, sw / S};
-    double Xn[3], F[3], drift = 0;
-#pragma unroll
-    for (int q = 0; q < 3; q++) {
-        const double x0 = a.x0[(size_t)q * n + k], xr = a.xref[(size_t)q * n + k];
-        Xn[q] = x0 + U[q] * a.dt;
-        F[q] = (-a.k_dA * (Xn[q] - xr));
-        drift = fmax(drift, fabs(Xn[q] - xr));
-    }
-    double S2 = 0;
-    for (int p = p0 + lane; p < p1; p += 64) {
-        const int c = a.np_cell[p];
-        const double d = ibm_delta(a.kernel, a.ih, a.cc, a.N, c, Xn);
-        if (d != 0.0) S2 += d * a.vol[c];
-    }
-    S2 = wave_allsum(S2);
-    if (lane == 0) {
-#pragma unroll
-        for (int q = 0; q < 3; q++) {
-            a.x[(size_t)q * n + k] = Xn[q];
-            a.x0[(size_t)q * n + k] = Xn[q];
-            a.vel[(size_t)q * n + k] = U[q];
-            a.force[(size_t)q * n + k] = F[q];
-        }
-        a.norm[k] = S2;
-        if (!(S > 0.0) || !(S2 > 0.0) || drift > a.margin) {
-            ctrl->status = PLBFS_ERR_IBM;
-            ctrl->done = 1;
-        }
-    }
-}
-
-__global__ __launch_bounds__(kBlock) void k_ibm_spread(IbmArgs a, const Ctrl *__restrict__ ctrl,
-                                                       double *__restrict__ fbody) {
-    if (ctrl->done) return;
-    const int j = blockIdx.x * kBlock + threadIdx.x;
-    if (j >= a.n_affected) return;
-    const int c = a.cp_cell[j];
-    const int n = a.n;
-    double f[3] = {0, 0, 0};
-    for (int p = a.cp_off[j]; p < a.cp_off[j + 1]; p++) {
-        const int k = a.cp_node[p];
-        const double X[3] = {a.x[k], a.x[(size_t)n + k], a.x[2 * (size_t)n + k]};
-        const double d = ibm_delta(a.kernel, a.ih, a.cc, a.N, c, X);
-        if (d == 0.0) continue;
-        const double q = d / a.norm[k];
-        f[0] += a.force[k] * q;
-        f[1] += a.force[(size_t)n + k] * q;
-        f[2] += a.force[2 * (size_t)n + k] * q;
-    }
-    fbody[c] = f[0];
-    fbody[(size_t)a.N + c] = f[1];
-    fbody[2 * (size_t)a.N + c] = f[2];
-}
 
 IbmArgs make_args(plbfs_ctx *c) {
     const IbmState &s = c->ibm;
     IbmArgs a;
     a.n = s.n;
     a.N = c->L.N;
+    a.B = 1;
+    a.node_kdA = nullptr;
+    a.node_case = nullptr;
+    a.cp_case = nullptr;
+    a.case_done = nullptr;
+    a.case_status = nullptr;
     a.kernel = s.kernel;
     a.ih = 1.0 / s.h;
     a.dt = s.dt;
@@ -177,28 +69,28 @@ int ibm_launch(plbfs_ctx *c) {
     TimedLaunch t(c, PLBFS_KSTAT_IBM);
     const dim3 grid((a.n + kNodesPerBlock - 1) / kNodesPerBlock);
     if (c->f32)
-        hipLaunchKernelGGL(k_ibm_nodes<float4>, grid, dim3(kBlock), 0, c->stream, a, c->ctrl, (const float4 *)c->stA);
+        hipLaunchKernelGGL(k_ibm_nodes<float4>, grid, dim3(kIbmThreads), 0, c->stream, a, c->ctrl, (const float4 *)c->stA);
     else
-        hipLaunchKernelGGL(k_ibm_nodes<double4>, grid, dim3(kBlock), 0, c->stream, a, c->ctrl,
+        hipLaunchKernelGGL(k_ibm_nodes<double4>, grid, dim3(kIbmThreads), 0, c->stream, a, c->ctrl,
                            (const double4 *)c->stA);
-    hipLaunchKernelGGL(k_ibm_spread, dim3((a.n_affected + kBlock - 1) / kBlock), dim3(kBlock), 0, c->stream, a,
+    hipLaunchKernelGGL(k_ibm_spread, dim3((a.n_affected + kIbmThreads - 1) / kIbmThreads), dim3(kIbmThreads), 0, c->stream, a,
                        c->ctrl, c->fbody);
     HIP_TRY(hipGetLastError());
     return PLBFS_OK;
 }
 
-extern "C" int plbfs_set_ibm(plbfs_ctx *c, const plbfs_ibm_nodes *nodes) {
-    if (!c || !nodes) return plbfs_fail(PLBFS_ERR_INPUT, "null argument");
+int build_ibm_pairs(const plbfs::Layout &L, const plbfs_ibm_nodes *nodes, IbmPairs &P) {
     const int n = nodes->n_nodes;
     if (n <= 0 || !nodes->x_ref || !nodes->x || !(nodes->h > 0) || nodes->node_area < 0 ||
         (nodes->kernel != PLBFS_DELTA_HAT2 && nodes->kernel != PLBFS_DELTA_PESKIN4))
         return plbfs_fail(PLBFS_ERR_INPUT, "bad immersed-boundary node set");
-    const int N = c->L.N;
+    const int N = L.N;
     const double h = nodes->h;
     const double support = (nodes->kernel == PLBFS_DELTA_HAT2 ? 1.0 : 2.0) * h;
     const double margin = 0.5 * h;
     const double reach = support + margin;
-    const double *cc = c->L.centroid.data();
+    P.margin = margin;
+    const double *cc = L.centroid.data();
     for (int k = 0; k < n; k++)
         for (int q = 0; q < 3; q++)
             if (std::fabs(nodes->x[3 * k + q] - nodes->x_ref[3 * k + q]) > margin)
@@ -223,7 +115,9 @@ extern "C" int plbfs_set_ibm(plbfs_ctx *c, const plbfs_ibm_nodes *nodes) {
                   (int64_t)std::floor((z - lo[2]) / reach))]
             .push_back(i);
     }
-    std::vector<int32_t> np_off(n + 1, 0), np_cell;
+    std::vector<int32_t> &np_off = P.np_off, &np_cell = P.np_cell;
+    np_off.assign(n + 1, 0);
+    np_cell.clear();
     std::vector<std::pair<int32_t, int32_t>> cell_node;  // (cell, node)
     for (int k = 0; k < n; k++) {
         const double *X = nodes->x_ref + 3 * k;
@@ -250,7 +144,10 @@ extern "C" int plbfs_set_ibm(plbfs_ctx *c, const plbfs_ibm_nodes *nodes) {
         np_off[k + 1] = (int32_t)np_cell.size();
     }
     std::sort(cell_node.begin(), cell_node.end());
-    std::vector<int32_t> cp_cell, cp_off(1, 0), cp_node;
+    std::vector<int32_t> &cp_cell = P.cp_cell, &cp_off = P.cp_off, &cp_node = P.cp_node;
+    cp_cell.clear();
+    cp_node.clear();
+    cp_off.assign(1, 0);
     for (size_t p = 0; p < cell_node.size(); p++) {
         if (p == 0 || cell_node[p].first != cell_node[p - 1].first) {
             if (p > 0) cp_off.push_back((int32_t)p);
@@ -259,6 +156,20 @@ extern "C" int plbfs_set_ibm(plbfs_ctx *c, const plbfs_ibm_nodes *nodes) {
         cp_node.push_back(cell_node[p].second);
     }
     cp_off.push_back((int32_t)cell_node.size());
+    return PLBFS_OK;
+}
+
+extern "C" int plbfs_set_ibm(plbfs_ctx *c, const plbfs_ibm_nodes *nodes) {
+    if (!c || !nodes) return plbfs_fail(PLBFS_ERR_INPUT, "null argument");
+    IbmPairs P;
+    int rc = build_ibm_pairs(c->L, nodes, P);
+    if (rc) return rc;
+    const int n = nodes->n_nodes;
+    const int N = c->L.N;
+    const double h = nodes->h;
+    const double margin = P.margin;
+    const std::vector<int32_t> &np_off = P.np_off, &np_cell = P.np_cell, &cp_cell = P.cp_cell, &cp_off = P.cp_off,
+                               &cp_node = P.cp_node;
     // device arrays
     IbmState &s = c->ibm;
     s = IbmState();
@@ -277,7 +188,6 @@ extern "C" int plbfs_set_ibm(plbfs_ctx *c, const plbfs_ibm_nodes *nodes) {
             xr[(size_t)q * n + k] = nodes->x_ref[3 * k + q];
             xx[(size_t)q * n + k] = nodes->x[3 * k + q];
         }
-    int rc;
     if ((rc = upload(c, &s.xref, xr)) || (rc = upload(c, &s.x, xx)) || (rc = upload(c, &s.x0, xx)) ||
         (rc = upload(c, &s.vel, z)) || (rc = upload(c, &s.force, z)) || (rc = upload(c, &s.norm, zn)) ||
         (rc = upload(c, &s.np_off, np_off)) || (rc = upload(c, &s.np_cell, np_cell)) ||

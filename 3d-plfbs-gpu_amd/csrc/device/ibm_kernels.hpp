@@ -1,0 +1,159 @@
+// Immersed-boundary kernels shared by the single-case solver (ibm.hip) and
+// the batched solver (batch.hip).
+//
+// Per step, before the RK stages (src/Solver_gpu.cu:687-718):
+//   k_ibm_nodes   one 64-wide wavefront per node: interpolate the node velocity
+//                 from the step-start state (src/immersed_boundary_method.cu:13-90),
+//                 move X = X0 + U dt (:244-295), spring force F = -k dA (X - X_ref)
+//                 (:161-190), and the spreading normaliser at the new position;
+//   k_ibm_spread  one thread per affected (cell, case) record: sum of the
+//                 node forces over its pairs in node order (:299-375).
+// Weights are normalised per node, W_cn = delta V_c / sum delta V, which is the
+// reference's formula on its uniform unit grid.  No atomics.
+//
+// For a batch, node k belongs to case node_case[k] and reads the state record
+// cell * B + case; affected records carry their case in cp_case.  Nodes and
+// records of a case whose case_done flag is set are skipped.
+#pragma once
+#include "lbfs_device.hpp"
+#include "plbfs_hip.h"
+
+namespace plbfs {
+namespace {
+
+constexpr int kIbmThreads = 256;
+constexpr int kNodesPerBlock = kIbmThreads / 64;
+
+__device__ __forceinline__ double ibm_phi(int kernel, double r) {
+    r = fabs(r);
+    if (kernel == PLBFS_DELTA_HAT2) return r < 1.0 ? 1.0 - r : 0.0;
+    // Peskin, "The immersed boundary method", Acta Numerica 11 (2002), 4-point function
+    if (r <= 1.0) return (3.0 - 2.0 * r + sqrt(1.0 + 4.0 * r - 4.0 * r * r)) / 8.0;
+    if (r <= 2.0) return (5.0 - 2.0 * r - sqrt(-7.0 + 12.0 * r - 4.0 * r * r)) / 8.0;
+    return 0.0;
+}
+
+__device__ __forceinline__ double ibm_delta(int kernel, double ih, const double *__restrict__ cc, int N, int c,
+                                            const double X[3]) {
+    return ibm_phi(kernel, (cc[c] - X[0]) * ih) * ibm_phi(kernel, (cc[(size_t)N + c] - X[1]) * ih) *
+           ibm_phi(kernel, (cc[2 * (size_t)N + c] - X[2]) * ih) * (ih * ih * ih);
+}
+
+__device__ __forceinline__ double wave_allsum(double x) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    return x;
+}
+
+struct IbmArgs {
+    int n, N, kernel;
+    int B;  // cases per cell record (1 for a single case)
+    double ih, dt, k_dA, margin;
+    double *x, *x0, *xref, *vel, *force, *norm;
+    const double *node_kdA;  // per-node k dA, or null for the scalar k_dA
+    const int32_t *node_case;  // per-node case, or null (case 0)
+    const int32_t *np_off, *np_cell;
+    const int32_t *cp_cell, *cp_off, *cp_node;
+    const int32_t *cp_case;  // per affected record case, or null (case 0)
+    const int32_t *case_done;  // per-case freeze flags, or null
+    int *case_status;          // per-case status, or null
+    int n_affected;
+    const double *cc, *vol;
+};
+
+template <typename SV>
+__global__ __launch_bounds__(kIbmThreads) void k_ibm_nodes(IbmArgs a, Ctrl *__restrict__ ctrl,
+                                                           const SV *__restrict__ st) {
+    if (ctrl->done) return;
+    const int lane = threadIdx.x & 63;
+    const int k = blockIdx.x * kNodesPerBlock + (threadIdx.x >> 6);
+    if (k >= a.n) return;
+    const int b = a.node_case ? a.node_case[k] : 0;
+    if (a.case_done && a.case_done[b]) return;
+    const int n = a.n;
+    const double X[3] = {a.x[k], a.x[(size_t)n + k], a.x[2 * (size_t)n + k]};
+    const int p0 = a.np_off[k], p1 = a.np_off[k + 1];
+    double S = 0, su = 0, sv = 0, sw = 0;
+    for (int p = p0 + lane; p < p1; p += 64) {
+        const int c = a.np_cell[p];
+        const double d = ibm_delta(a.kernel, a.ih, a.cc, a.N, c, X);
+        if (d != 0.0) {
+            const double wv = d * a.vol[c];
+            const SV s = st[(size_t)c * a.B + b];
+            S += wv;
+            su += wv * s.y;
+            sv += wv * s.z;
+            sw += wv * s.w;
+        }
+    }
+    S = wave_allsum(S);
+    su = wave_allsum(su);
+    sv = wave_allsum(sv);
+    sw = wave_allsum(sw);
+    const double U[3] = {su / S, sv / S, sw / S};
+    const double kdA = a.node_kdA ? a.node_kdA[k] : a.k_dA;
+    double Xn[3], F[3], drift = 0;
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        const double x0 = a.x0[(size_t)q * n + k], xr = a.xref[(size_t)q * n + k];
+        Xn[q] = x0 + U[q] * a.dt;
+        F[q] = (-kdA * (Xn[q] - xr));
+        drift = fmax(drift, fabs(Xn[q] - xr));
+    }
+    double S2 = 0;
+    for (int p = p0 + lane; p < p1; p += 64) {
+        const int c = a.np_cell[p];
+        const double d = ibm_delta(a.kernel, a.ih, a.cc, a.N, c, Xn);
+        if (d != 0.0) S2 += d * a.vol[c];
+    }
+    S2 = wave_allsum(S2);
+    if (lane == 0) {
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            a.x[(size_t)q * n + k] = Xn[q];
+            a.x0[(size_t)q * n + k] = Xn[q];
+            a.vel[(size_t)q * n + k] = U[q];
+            a.force[(size_t)q * n + k] = F[q];
+        }
+        a.norm[k] = S2;
+        if (!(S > 0.0) || !(S2 > 0.0) || drift > a.margin) {
+            if (a.case_status) {
+                // batch: only this case stops
+                a.case_status[b] = PLBFS_ERR_IBM;
+            } else {
+                ctrl->status = PLBFS_ERR_IBM;
+                ctrl->done = 1;
+            }
+        }
+    }
+}
+
+// fbody is [3][B*N] component-major over records.
+__global__ __launch_bounds__(kIbmThreads) void k_ibm_spread(IbmArgs a, const Ctrl *__restrict__ ctrl,
+                                                            double *__restrict__ fbody) {
+    if (ctrl->done) return;
+    const int j = blockIdx.x * kIbmThreads + threadIdx.x;
+    if (j >= a.n_affected) return;
+    const int c = a.cp_cell[j];
+    const int b = a.cp_case ? a.cp_case[j] : 0;
+    if (a.case_done && a.case_done[b]) return;
+    const int n = a.n;
+    double f[3] = {0, 0, 0};
+    for (int p = a.cp_off[j]; p < a.cp_off[j + 1]; p++) {
+        const int k = a.cp_node[p];
+        const double X[3] = {a.x[k], a.x[(size_t)n + k], a.x[2 * (size_t)n + k]};
+        const double d = ibm_delta(a.kernel, a.ih, a.cc, a.N, c, X);
+        if (d == 0.0) continue;
+        const double q = d / a.norm[k];
+        f[0] += a.force[k] * q;
+        f[1] += a.force[(size_t)n + k] * q;
+        f[2] += a.force[2 * (size_t)n + k] * q;
+    }
+    const size_t NR = (size_t)a.N * a.B, r = (size_t)c * a.B + b;
+    fbody[r] = f[0];
+    fbody[NR + r] = f[1];
+    fbody[2 * NR + r] = f[2];
+}
+
+}  // namespace
+}  // namespace plbfs

@@ -26,6 +26,7 @@
 #include <vector>
 
 #include "ctx.hpp"
+#include "stage.hpp"
 
 using namespace plbfs;
 
@@ -37,182 +38,21 @@ constexpr int kFinalizeThreads = 256;
 // Kernels
 // ---------------------------------------------------------------------------
 
+// Occupancy 8 waves/SIMD (<= 64 VGPRs) keeps enough gathers in flight; the
+// compiler otherwise settles at 92 VGPRs (5 waves) for the FP64 instance.
 template <typename S>
-__global__ __launch_bounds__(kBlock) void k_grad(DevMesh<S> m, const Ctrl *__restrict__ ctrl,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) void k_grad(DevMesh<S> m, const Ctrl *__restrict__ ctrl,
                                                  const V4<S> *__restrict__ st, V4<S> *__restrict__ grad) {
     if (ctrl->done) return;
-    const int N = m.N;
     const int i = xcd_block(blockIdx.x, gridDim.x, m.xcd_remap) * kBlock + threadIdx.x;
-    if (i >= N) return;
-    const V4<S> q0 = st[i];
-    const S p0[4] = {q0.x, q0.y, q0.z, q0.w};
+    if (i >= m.N) return;
     S g[12];
-#pragma unroll
-    for (int q = 0; q < 12; q++) g[q] = (S)0;
-    const int cnt = m.st_cnt[i];
-#pragma unroll
-    for (int s = 0; s < kMaxSlots; s++) {
-        if (s < cnt) {
-            const int j = m.st_nbr[(size_t)s * N + i];
-            S pj[4];
-            if (j >= 0) {
-                const V4<S> qj = st[j];
-                pj[0] = qj.x;
-                pj[1] = qj.y;
-                pj[2] = qj.z;
-                pj[3] = qj.w;
-            } else {
-                const DevPatch P = m.patches[m.b_patch[-1 - j]];
-                ghost_state<S>(P, p0, pj);
-            }
-            const S wx = m.st_w[((size_t)s * 3 + 0) * N + i];
-            const S wy = m.st_w[((size_t)s * 3 + 1) * N + i];
-            const S wz = m.st_w[((size_t)s * 3 + 2) * N + i];
-#pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const S d = pj[q] - p0[q];
-                g[3 * q] += wx * d;
-                g[3 * q + 1] += wy * d;
-                g[3 * q + 2] += wz * d;
-            }
-        }
-    }
+    lsq_grad<S>(m, i, st, Rec1{}, g);
     V4<S> *gi = grad + (size_t)3 * i;
     gi[0] = V4<S>{g[0], g[1], g[2], g[3]};
     gi[1] = V4<S>{g[4], g[5], g[6], g[7]};
     gi[2] = V4<S>{g[8], g[9], g[10], g[11]};
 }
-
-template <typename S>
-__device__ __forceinline__ void load_side(const V4<S> *__restrict__ st, const V4<S> *__restrict__ grad, int c,
-                                          S p[4], S g[12]) {
-    const V4<S> q = st[c];
-    p[0] = q.x;
-    p[1] = q.y;
-    p[2] = q.z;
-    p[3] = q.w;
-    const V4<S> *gc = grad + (size_t)3 * c;
-    const V4<S> a = gc[0], b = gc[1], d = gc[2];
-    g[0] = a.x; g[1] = a.y; g[2] = a.z; g[3] = a.w;
-    g[4] = b.x; g[5] = b.y; g[6] = b.z; g[7] = b.w;
-    g[8] = d.x; g[9] = d.y; g[10] = d.z; g[11] = d.w;
-}
-
-// Flux of face work item e (src/Solver.cpp:286-347 with populate_lattice_macros :840-1274).
-template <typename S>
-__device__ __forceinline__ void item_flux(const DevMesh<S> &m, const DevParams &prm, int64_t e,
-                                          const V4<S> *__restrict__ st, const V4<S> *__restrict__ grad, S F[4]) {
-    const int64_t E = m.E;
-    const int o = m.it_own[e];
-    const int nb = m.it_nbr[e];
-    S geo[kGeoPerItem];  // n(3), A, dt, r_owner(3), r_neighbour(3)
-    if constexpr (std::is_same<S, double>::value) {
-#pragma unroll
-        for (int q = 0; q < 5; q++) geo[q] = m.it_geo[(size_t)q * E + e];
-        const double fx = m.it_fc[e], fy = m.it_fc[(size_t)E + e], fz = m.it_fc[2 * (size_t)E + e];
-        const double4 co = m.cgeo[o];
-        geo[5] = co.x - fx;
-        geo[6] = co.y - fy;
-        geo[7] = co.z - fz;
-        if (nb >= 0) {
-            const double4 cn = m.cgeo[nb];
-            geo[8] = cn.x - fx;
-            geo[9] = cn.y - fy;
-            geo[10] = cn.z - fz;
-        } else {
-            geo[8] = m.g0x - fx;
-            geo[9] = m.g0y - fy;
-            geo[10] = m.g0z - fz;
-        }
-    } else {
-#pragma unroll
-        for (int q = 0; q < kGeoPerItem; q++) geo[q] = m.it_geo[(size_t)q * E + e];
-    }
-    const S n[3] = {geo[0], geo[1], geo[2]};
-    const S A = geo[3], dt = geo[4];
-    S po[4], go[12];
-    load_side<S>(st, grad, o, po, go);
-    S pI[4], pN[4], p0[4], gN[12];
-    if (nb >= 0 || nb == -1) {
-        // interior branch: a real neighbour, or the ghost whose index equals
-        // n_cells (the reference's strict "neighbour > n_cells" test, Solver.cpp:861)
-        S pn[4];
-        if (nb >= 0) {
-            load_side<S>(st, grad, nb, pn, gN);
-        } else {
-            const DevPatch P = m.patches[m.b_patch[0]];
-            ghost_state<S>(P, po, pn);
-            ghost_grad<S>(P, n, m.b_dmag[0], po, go, gN);
-        }
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            pI[q] = po[q] - (geo[5] * go[3 * q] + geo[6] * go[3 * q + 1] + geo[7] * go[3 * q + 2]);
-            pN[q] = pn[q] - (geo[8] * gN[3 * q] + geo[9] * gN[3 * q + 1] + geo[10] * gN[3 * q + 2]);
-            p0[q] = (pI[q] + pN[q]) * (S)0.5;
-        }
-    } else {
-        const int b = -1 - nb;
-        const DevPatch P = m.patches[m.b_patch[b]];
-        ghost_grad<S>(P, n, m.b_dmag[b], po, go, gN);
-        S fv[4];
-        if (P.rho_type == 1) fv[0] = (S)P.rho;
-        else if (P.rho_type == 8) fv[0] = po[0];
-        else fv[0] = po[0] - (geo[5] * gN[0] + geo[6] * gN[1] + geo[7] * gN[2]);
-        if (P.vel_type == 1) {
-            fv[1] = (S)P.u;
-            fv[2] = (S)P.v;
-            fv[3] = (S)P.w;
-        } else if (P.vel_type == 8) {
-            fv[1] = po[1];
-            fv[2] = (S)0;
-            fv[3] = po[3];
-        } else {
-#pragma unroll
-            for (int q = 1; q < 4; q++) fv[q] = po[q] - (geo[5] * gN[3 * q] + geo[6] * gN[3 * q + 1] + geo[7] * gN[3 * q + 2]);
-        }
-#pragma unroll
-        for (int q = 0; q < 4; q++) pI[q] = pN[q] = p0[q] = fv[q];
-    }
-    const S tau = (S)prm.visc3_invgam / dt;
-    lattice_flux<std::is_same<S, float>::value, S>(n, A, dt, tau, m.it_side[e], p0, pI, go, pN, gN, (S)prm.invgam,
-                                                   (S)prm.rho0, F);
-}
-
-// Per-cell RK accumulator between stages: the running sum t1 itself in FP64
-// storage; in FP32 storage only the increment sum_k beta_k dt R_k (the
-// conservative state stays in t0 in FP64, so small increments keep their bits).
-template <typename S>
-struct AccT {
-    using type = double4;
-};
-template <>
-struct AccT<float> {
-    using type = float4;
-};
-template <typename S>
-using Acc = typename AccT<S>::type;
-
-__device__ __forceinline__ double local_dt(const DevParams &prm, double rho, double u, double ax, double vol) {
-    // Solver.cpp:1715-1744 with the x-slot-only CFL areas (A_y = A_z = 0)
-    double eig = (fabs(u) + prm.ceff) * ax;
-    eig = eig / prm.gamma;
-    double ve = 2 * prm.visc / prm.gamma / rho / vol;
-    ve = ve * (ax * ax);
-    eig = eig + 4 * ve;
-    return prm.cfl * vol / eig;
-}
-
-template <typename T>
-__device__ __forceinline__ T wave_sum(T x) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
-    return x;
-}
-
-struct StageArgs {
-    double alpha, beta;
-    int stage;  // 0..3
-};
 
 template <typename S>
 __global__ __launch_bounds__(kBlock) void k_flux_update(DevMesh<S> m, DevParams prm, StageArgs sa,
@@ -222,7 +62,6 @@ __global__ __launch_bounds__(kBlock) void k_flux_update(DevMesh<S> m, DevParams 
                                                         double4 *__restrict__ t0, Acc<S> *__restrict__ t1,
                                                         double *__restrict__ dtl, const double *__restrict__ fbody,
                                                         double *__restrict__ partials) {
-    constexpr bool F32 = std::is_same<S, float>::value;
     extern __shared__ double lds[];
     if (ctrl->done) return;
     const int N = m.N;
@@ -232,7 +71,7 @@ __global__ __launch_bounds__(kBlock) void k_flux_update(DevMesh<S> m, DevParams 
     S *sF = reinterpret_cast<S *>(lds);
     for (int e = e0 + threadIdx.x; e < e1; e += kBlock) {
         S F[4];
-        item_flux<S>(m, prm, e, st_in, grad, F);
+        item_flux<S>(m, prm, e, st_in, grad, Rec1{}, prm.visc3_invgam, F);
         const int sl = e - e0;
 #pragma unroll
         for (int q = 0; q < 4; q++) sF[q * nslot + sl] = F[q];
@@ -242,93 +81,16 @@ __global__ __launch_bounds__(kBlock) void k_flux_update(DevMesh<S> m, DevParams 
     int err = 0;
     const int c_end = min(N, (blk + 1) * m.block_cells);
     for (int i = blk * m.block_cells + threadIdx.x; i < c_end; i += kBlock) {
-        const double iv = m.inv_vol[i];
-        double R[4] = {0, 0, 0, 0};
-        const int k0 = m.g_off[i], k1 = m.g_off[i + 1];
-        for (int k = k0; k < k1; k++) {
-            const int ent = m.g_ent[k];
-            const int sl = ent >> 1;
-            if (ent & 1) {
-#pragma unroll
-                for (int q = 0; q < 4; q++) R[q] = R[q] + (-(double)sF[q * nslot + sl] * iv);
-            } else {
-#pragma unroll
-                for (int q = 0; q < 4; q++) R[q] = R[q] + ((double)sF[q * nslot + sl] * iv);
-            }
-        }
-        if (F32) {
-            // rho0/3 sum(+-n A)/V of the cell, omitted from the single-precision face fluxes
-            R[1] += (double)m.corr[i];
-            R[2] += (double)m.corr[(size_t)N + i];
-            R[3] += (double)m.corr[2 * (size_t)N + i];
-        }
+        double R[4];
+        gather_residual<S>(m, i, sF, nslot, 1, 0, R);
         if (prm.use_body_force) {
             R[1] += fbody[i];
             R[2] += fbody[(size_t)N + i];
             R[3] += fbody[2 * (size_t)N + i];
         }
-        const double dti = dtl[i];
-        const double4 ta = t0[i];
-        const double a0 = ta.x, a1 = ta.y, a2 = ta.z, a3 = ta.w;
-        // Solver.cpp:412-416
-        double f1 = a0 + R[0] * dti * sa.alpha;
-        double f2 = a1 + (R[1] + prm.force) * dti * sa.alpha;
-        double f3 = a2 + R[2] * dti * sa.alpha;
-        double f4 = a3 + R[3] * dti * sa.alpha;
-        f2 = f2 / f1;
-        f3 = f3 / f1;
-        f4 = f4 / f1;
-        // Solver.cpp:424-428
-        const double db = dti * sa.beta;
-        double b0, b1, b2, b3;
-        if (!F32) {
-            if (sa.stage == 0) {
-                b0 = a0; b1 = a1; b2 = a2; b3 = a3;
-            } else {
-                const Acc<S> tb = t1[i];
-                b0 = tb.x; b1 = tb.y; b2 = tb.z; b3 = tb.w;
-            }
-            b0 = b0 + db * R[0];
-            b1 = b1 + db * (R[1] + prm.force);
-            b2 = b2 + db * R[2];
-            b3 = b3 + db * R[3];
-        } else {
-            // increments only; the conservative state is t0 + increments
-            double d0 = 0, d1 = 0, d2 = 0, d3 = 0;
-            if (sa.stage != 0) {
-                const Acc<S> tb = t1[i];
-                d0 = tb.x; d1 = tb.y; d2 = tb.z; d3 = tb.w;
-            }
-            d0 += db * R[0];
-            d1 += db * (R[1] + prm.force);
-            d2 += db * R[2];
-            d3 += db * R[3];
-            if (sa.stage < 3) {
-                b0 = d0; b1 = d1; b2 = d2; b3 = d3;
-            } else {
-                b0 = a0 + d0; b1 = a1 + d1; b2 = a2 + d2; b3 = a3 + d3;
-            }
-        }
-        const double r0 = F32 ? prm.rho0 : 0.0;  // single-precision density is stored as rho - rho0
-        if (sa.stage < 3) {
-            st_out[i] = V4<S>{(S)(f1 - r0), (S)f2, (S)f3, (S)f4};
-            using E = typename std::conditional<F32, float, double>::type;
-            t1[i] = Acc<S>{(E)b0, (E)b1, (E)b2, (E)b3};
-        } else {
-            // soln = t1 as primitives (Solver.cpp:431-436); it is the next step's temp
-            const double s0 = b0, s1 = b1 / b0, s2 = b2 / b0, s3 = b3 / b0;
-            st_out[i] = V4<S>{(S)(s0 - r0), (S)s1, (S)s2, (S)s3};
-            t0[i] = double4{b0, b1, b2, b3};
-            if (prm.time_stepping == PLBFS_DT_CONSTANT) dtl[i] = prm.cfl;
-            else dtl[i] = local_dt(prm, s0, s1, m.cfl_area[i], m.vol[i]);
-            sq[0] += R[0] * R[0];
-            sq[1] += R[1] * R[1];
-            sq[2] += R[2] * R[2];
-            sq[3] += R[3] * R[3];
-            // Solver.cpp:448-493 guards on the stage state
-            if (isnan(f1) || isnan(f2)) err = 1;
-            else if (f1 / prm.ref_rho > 1000.0 && err == 0) err = 2;
-        }
+        const int e = rk_update<S>(m, prm, prm.visc, sa, i, (size_t)i, R, st_out, t0, t1, dtl, sq);
+        if (e == 1) err = 1;
+        else if (e == 2 && err == 0) err = 2;
     }
     if (sa.stage == 3) {
         __shared__ double red[kBlock / 64][5];
@@ -400,7 +162,7 @@ __global__ __launch_bounds__(kBlock) void k_init_dt(int N, DevParams prm, const 
     if (i >= N) return;
     const double r0 = std::is_same<S, float>::value ? prm.rho0 : 0.0;
     if (prm.time_stepping == PLBFS_DT_CONSTANT) dtl[i] = prm.cfl;
-    else dtl[i] = local_dt(prm, r0 + (double)st[i].x, (double)st[i].y, cfl_area[i], vol[i]);
+    else dtl[i] = local_dt(prm, prm.visc, r0 + (double)st[i].x, (double)st[i].y, cfl_area[i], vol[i]);
 }
 
 __global__ __launch_bounds__(kFinalizeThreads) void k_finalize(int nblk, double n_cells, double ref_rho,
@@ -464,28 +226,15 @@ __global__ __launch_bounds__(kBlock) void k_residual_only(DevMesh<S> m, DevParam
     S *sF = reinterpret_cast<S *>(lds);
     for (int e = e0 + threadIdx.x; e < e1; e += kBlock) {
         S F[4];
-        item_flux<S>(m, prm, e, st_in, grad, F);
+        item_flux<S>(m, prm, e, st_in, grad, Rec1{}, prm.visc3_invgam, F);
 #pragma unroll
         for (int q = 0; q < 4; q++) sF[q * nslot + (e - e0)] = F[q];
     }
     __syncthreads();
     const int c_end = min(N, (blk + 1) * m.block_cells);
     for (int i = blk * m.block_cells + threadIdx.x; i < c_end; i += kBlock) {
-        const double iv = m.inv_vol[i];
-        double R[4] = {0, 0, 0, 0};
-        for (int k = m.g_off[i]; k < m.g_off[i + 1]; k++) {
-            const int ent = m.g_ent[k];
-            const int sl = ent >> 1;
-            for (int q = 0; q < 4; q++) {
-                const double v = (double)sF[q * nslot + sl] * iv;
-                R[q] = R[q] + ((ent & 1) ? -v : v);
-            }
-        }
-        if (std::is_same<S, float>::value) {
-            R[1] += (double)m.corr[i];
-            R[2] += (double)m.corr[(size_t)N + i];
-            R[3] += (double)m.corr[2 * (size_t)N + i];
-        }
+        double R[4];
+        gather_residual<S>(m, i, sF, nslot, 1, 0, R);
         if (prm.use_body_force) {
             R[1] += fbody[i];
             R[2] += fbody[(size_t)N + i];
@@ -523,6 +272,17 @@ extern "C" int plbfs_device_count(void) {
 
 extern "C" int plbfs_create(const plbfs_mesh_view *mesh, const plbfs_params *params, const plbfs_patch *patches,
                             int32_t n_patches, const int32_t *bface_patch, plbfs_ctx **out) {
+    // cells per flux block: a multiple of the 256-thread block; larger blocks
+    // duplicate fewer border faces but need more LDS
+    int block_cells = kFluxBlockCells;
+    if (const char *bc = std::getenv("PLBFS_BLOCK_CELLS")) block_cells = std::atoi(bc);
+    if (block_cells < kBlock || block_cells % kBlock || block_cells > 4 * kBlock)
+        return plbfs_fail(PLBFS_ERR_INPUT, "PLBFS_BLOCK_CELLS must be a multiple of %d up to %d", kBlock, 4 * kBlock);
+    return create_ctx(mesh, params, patches, n_patches, bface_patch, block_cells, 1, out);
+}
+
+int create_ctx(const plbfs_mesh_view *mesh, const plbfs_params *params, const plbfs_patch *patches,
+               int32_t n_patches, const int32_t *bface_patch, int block_cells, int lds_cases, plbfs_ctx **out) {
     if (!mesh || !params || !out) return plbfs_fail(PLBFS_ERR_INPUT, "null argument");
     *out = nullptr;
     const plbfs_params &p = *params;
@@ -537,14 +297,6 @@ extern "C" int plbfs_create(const plbfs_mesh_view *mesh, const plbfs_params *par
     c->params = p;
     c->f32 = p.precision == PLBFS_FP32;
     c->patches.assign(patches, patches + n_patches);
-    // cells per flux block: a multiple of the 256-thread block; larger blocks
-    // duplicate fewer border faces but need more LDS
-    int block_cells = kFluxBlockCells;
-    if (const char *bc = std::getenv("PLBFS_BLOCK_CELLS")) block_cells = std::atoi(bc);
-    if (block_cells < kBlock || block_cells % kBlock || block_cells > 4 * kBlock) {
-        delete c;
-        return plbfs_fail(PLBFS_ERR_INPUT, "PLBFS_BLOCK_CELLS must be a multiple of %d up to %d", kBlock, 4 * kBlock);
-    }
     int rc = build_layout(*mesh, p, patches, n_patches, bface_patch, block_cells, c->L);
     if (rc) {
         delete c;
@@ -552,11 +304,11 @@ extern "C" int plbfs_create(const plbfs_mesh_view *mesh, const plbfs_params *par
     }
     const Layout &L = c->L;
     const size_t sbytes = c->f32 ? sizeof(float) : sizeof(double);
-    c->lds_bytes = (size_t)L.max_items_per_block * 4 * sbytes;
-    if (c->lds_bytes > 64 * 1024) {
+    c->lds_bytes = (size_t)L.max_items_per_block * 4 * sbytes * lds_cases;
+    if (c->lds_bytes > kMaxLds) {
         delete c;
-        return plbfs_fail(PLBFS_ERR_INPUT, "a block needs %zu B of LDS for %d face slots (limit 64 KiB)",
-                          c->lds_bytes, L.max_items_per_block);
+        return plbfs_fail(PLBFS_ERR_INPUT, "a block needs %zu B of LDS for %d face slots x %d cases (limit %d B)",
+                          c->lds_bytes, L.max_items_per_block, lds_cases, kMaxLds);
     }
     auto fail = [&](int r) {
         release(c);

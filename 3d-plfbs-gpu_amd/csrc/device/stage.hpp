@@ -1,0 +1,299 @@
+// Per-cell and per-face pieces of one RK stage, shared by the single-case
+// kernels (solver.hip) and the batched kernels (batch.hip).
+//
+// A "record index" names where cell c's state and gradient live: c for one
+// case, c * B + b for case b of a batch stored case-fastest (so the B cases of
+// one cell are adjacent and a gather of one cell for consecutive cases is one
+// contiguous run).
+#pragma once
+#include <type_traits>
+
+#include "lbfs_device.hpp"
+#include "plbfs_hip.h"
+
+namespace plbfs {
+
+struct Rec1 {
+    __device__ __forceinline__ size_t operator()(int c) const { return (size_t)c; }
+};
+struct RecB {
+    int B, b;
+    __device__ __forceinline__ size_t operator()(int c) const { return (size_t)c * B + b; }
+};
+
+// Per-cell RK accumulator between stages: the running sum t1 itself in FP64
+// storage; in FP32 storage only the increment sum_k beta_k dt R_k (the
+// conservative state stays in t0 in FP64, so small increments keep their bits).
+template <typename S>
+struct AccT {
+    using type = double4;
+};
+template <>
+struct AccT<float> {
+    using type = float4;
+};
+template <typename S>
+using Acc = typename AccT<S>::type;
+
+struct StageArgs {
+    double alpha, beta;
+    int stage;  // 0..3
+};
+
+__device__ __forceinline__ double local_dt(const DevParams &prm, double visc, double rho, double u, double ax,
+                                           double vol) {
+    // Solver.cpp:1715-1744 with the x-slot-only CFL areas (A_y = A_z = 0)
+    double eig = (fabs(u) + prm.ceff) * ax;
+    eig = eig / prm.gamma;
+    double ve = 2 * visc / prm.gamma / rho / vol;
+    ve = ve * (ax * ax);
+    eig = eig + 4 * ve;
+    return prm.cfl * vol / eig;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T x) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    return x;
+}
+
+// Least-squares gradient of cell i (src/gradients.cpp:319-389): folded stencil
+// weights times neighbour differences; ghost neighbours from the owner's patch.
+template <typename S, typename R>
+__device__ __forceinline__ void lsq_grad(const DevMesh<S> &m, int i, const V4<S> *__restrict__ st, R rec, S g[12]) {
+    const int N = m.N;
+    const V4<S> q0 = st[rec(i)];
+    const S p0[4] = {q0.x, q0.y, q0.z, q0.w};
+#pragma unroll
+    for (int q = 0; q < 12; q++) g[q] = (S)0;
+    const int cnt = m.st_cnt[i];
+#pragma unroll
+    for (int s = 0; s < kMaxSlots; s++) {
+        if (s < cnt) {
+            const int j = m.st_nbr[(size_t)s * N + i];
+            S pj[4];
+            if (j >= 0) {
+                const V4<S> qj = st[rec(j)];
+                pj[0] = qj.x;
+                pj[1] = qj.y;
+                pj[2] = qj.z;
+                pj[3] = qj.w;
+            } else {
+                const DevPatch P = m.patches[m.b_patch[-1 - j]];
+                ghost_state<S>(P, p0, pj);
+            }
+            const S wx = m.st_w[((size_t)s * 3 + 0) * N + i];
+            const S wy = m.st_w[((size_t)s * 3 + 1) * N + i];
+            const S wz = m.st_w[((size_t)s * 3 + 2) * N + i];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const S d = pj[q] - p0[q];
+                g[3 * q] += wx * d;
+                g[3 * q + 1] += wy * d;
+                g[3 * q + 2] += wz * d;
+            }
+        }
+    }
+}
+
+template <typename S>
+__device__ __forceinline__ void load_side(const V4<S> *__restrict__ st, const V4<S> *__restrict__ grad, size_t r,
+                                          S p[4], S g[12]) {
+    const V4<S> q = st[r];
+    p[0] = q.x;
+    p[1] = q.y;
+    p[2] = q.z;
+    p[3] = q.w;
+    const V4<S> *gc = grad + 3 * r;
+    const V4<S> a = gc[0], b = gc[1], d = gc[2];
+    g[0] = a.x; g[1] = a.y; g[2] = a.z; g[3] = a.w;
+    g[4] = b.x; g[5] = b.y; g[6] = b.z; g[7] = b.w;
+    g[8] = d.x; g[9] = d.y; g[10] = d.z; g[11] = d.w;
+}
+
+// Flux of face work item e (src/Solver.cpp:286-347 with populate_lattice_macros
+// :840-1274); v3g = 3 nu / gamma of the case.
+template <typename S, typename R>
+__device__ __forceinline__ void item_flux(const DevMesh<S> &m, const DevParams &prm, int64_t e,
+                                          const V4<S> *__restrict__ st, const V4<S> *__restrict__ grad, R rec,
+                                          double v3g, S F[4]) {
+    const int64_t E = m.E;
+    const int o = m.it_own[e];
+    const int nb = m.it_nbr[e];
+    S geo[kGeoPerItem];  // n(3), A, dt, r_owner(3), r_neighbour(3)
+    if constexpr (std::is_same<S, double>::value) {
+#pragma unroll
+        for (int q = 0; q < 5; q++) geo[q] = m.it_geo[(size_t)q * E + e];
+        const double fx = m.it_fc[e], fy = m.it_fc[(size_t)E + e], fz = m.it_fc[2 * (size_t)E + e];
+        const double4 co = m.cgeo[o];
+        geo[5] = co.x - fx;
+        geo[6] = co.y - fy;
+        geo[7] = co.z - fz;
+        if (nb >= 0) {
+            const double4 cn = m.cgeo[nb];
+            geo[8] = cn.x - fx;
+            geo[9] = cn.y - fy;
+            geo[10] = cn.z - fz;
+        } else {
+            geo[8] = m.g0x - fx;
+            geo[9] = m.g0y - fy;
+            geo[10] = m.g0z - fz;
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < kGeoPerItem; q++) geo[q] = m.it_geo[(size_t)q * E + e];
+    }
+    const S n[3] = {geo[0], geo[1], geo[2]};
+    const S A = geo[3], dt = geo[4];
+    S po[4], go[12];
+    load_side<S>(st, grad, rec(o), po, go);
+    S pI[4], pN[4], p0[4], gN[12];
+    if (nb >= 0 || nb == -1) {
+        // interior branch: a real neighbour, or the ghost whose index equals
+        // n_cells (the reference's strict "neighbour > n_cells" test, Solver.cpp:861)
+        S pn[4];
+        if (nb >= 0) {
+            load_side<S>(st, grad, rec(nb), pn, gN);
+        } else {
+            const DevPatch P = m.patches[m.b_patch[0]];
+            ghost_state<S>(P, po, pn);
+            ghost_grad<S>(P, n, m.b_dmag[0], po, go, gN);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            pI[q] = po[q] - (geo[5] * go[3 * q] + geo[6] * go[3 * q + 1] + geo[7] * go[3 * q + 2]);
+            pN[q] = pn[q] - (geo[8] * gN[3 * q] + geo[9] * gN[3 * q + 1] + geo[10] * gN[3 * q + 2]);
+            p0[q] = (pI[q] + pN[q]) * (S)0.5;
+        }
+    } else {
+        const int b = -1 - nb;
+        const DevPatch P = m.patches[m.b_patch[b]];
+        ghost_grad<S>(P, n, m.b_dmag[b], po, go, gN);
+        S fv[4];
+        if (P.rho_type == 1) fv[0] = (S)P.rho;
+        else if (P.rho_type == 8) fv[0] = po[0];
+        else fv[0] = po[0] - (geo[5] * gN[0] + geo[6] * gN[1] + geo[7] * gN[2]);
+        if (P.vel_type == 1) {
+            fv[1] = (S)P.u;
+            fv[2] = (S)P.v;
+            fv[3] = (S)P.w;
+        } else if (P.vel_type == 8) {
+            fv[1] = po[1];
+            fv[2] = (S)0;
+            fv[3] = po[3];
+        } else {
+#pragma unroll
+            for (int q = 1; q < 4; q++) fv[q] = po[q] - (geo[5] * gN[3 * q] + geo[6] * gN[3 * q + 1] + geo[7] * gN[3 * q + 2]);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) pI[q] = pN[q] = p0[q] = fv[q];
+    }
+    const S tau = (S)v3g / dt;
+    lattice_flux<std::is_same<S, float>::value, S>(n, A, dt, tau, m.it_side[e], p0, pI, go, pN, gN, (S)prm.invgam,
+                                                   (S)prm.rho0, F);
+}
+
+// Residual of cell i from its faces' LDS slots, summed in the reference's face
+// order (src/Solver.cpp:353-366).  Slot sl of lane c lives at sF[q*stride + sl*C + c].
+template <typename S>
+__device__ __forceinline__ void gather_residual(const DevMesh<S> &m, int i, const S *sF, int stride, int C, int c,
+                                                double R[4]) {
+    const double iv = m.inv_vol[i];
+    R[0] = R[1] = R[2] = R[3] = 0;
+    const int k0 = m.g_off[i], k1 = m.g_off[i + 1];
+    for (int k = k0; k < k1; k++) {
+        const int ent = m.g_ent[k];
+        const int sl = (ent >> 1) * C + c;
+        if (ent & 1) {
+#pragma unroll
+            for (int q = 0; q < 4; q++) R[q] = R[q] + (-(double)sF[q * stride + sl] * iv);
+        } else {
+#pragma unroll
+            for (int q = 0; q < 4; q++) R[q] = R[q] + ((double)sF[q * stride + sl] * iv);
+        }
+    }
+    if constexpr (std::is_same<S, float>::value) {
+        // rho0/3 sum(+-n A)/V of the cell, omitted from the single-precision face fluxes
+        const int N = m.N;
+        R[1] += (double)m.corr[i];
+        R[2] += (double)m.corr[(size_t)N + i];
+        R[3] += (double)m.corr[2 * (size_t)N + i];
+    }
+}
+
+// RK stage update of one cell record r (cell i) from its residual R
+// (src/Solver.cpp:405-438, src/RungeKutta.h:16-18).  The last stage writes the
+// new primitive solution, t0 and the next local time step, adds R^2 to sq and
+// returns 1 for NaN or 2 for rho blow-up (src/Solver.cpp:448-493).
+template <typename S>
+__device__ __forceinline__ int rk_update(const DevMesh<S> &m, const DevParams &prm, double visc, StageArgs sa,
+                                         int i, size_t r, const double R[4], V4<S> *__restrict__ st_out,
+                                         double4 *__restrict__ t0, Acc<S> *__restrict__ t1,
+                                         double *__restrict__ dtl, double sq[4]) {
+    constexpr bool F32 = std::is_same<S, float>::value;
+    const double dti = dtl[r];
+    const double4 ta = t0[r];
+    const double a0 = ta.x, a1 = ta.y, a2 = ta.z, a3 = ta.w;
+    // Solver.cpp:412-416
+    double f1 = a0 + R[0] * dti * sa.alpha;
+    double f2 = a1 + (R[1] + prm.force) * dti * sa.alpha;
+    double f3 = a2 + R[2] * dti * sa.alpha;
+    double f4 = a3 + R[3] * dti * sa.alpha;
+    f2 = f2 / f1;
+    f3 = f3 / f1;
+    f4 = f4 / f1;
+    // Solver.cpp:424-428
+    const double db = dti * sa.beta;
+    double b0, b1, b2, b3;
+    if (!F32) {
+        if (sa.stage == 0) {
+            b0 = a0; b1 = a1; b2 = a2; b3 = a3;
+        } else {
+            const Acc<S> tb = t1[r];
+            b0 = tb.x; b1 = tb.y; b2 = tb.z; b3 = tb.w;
+        }
+        b0 = b0 + db * R[0];
+        b1 = b1 + db * (R[1] + prm.force);
+        b2 = b2 + db * R[2];
+        b3 = b3 + db * R[3];
+    } else {
+        // increments only; the conservative state is t0 + increments
+        double d0 = 0, d1 = 0, d2 = 0, d3 = 0;
+        if (sa.stage != 0) {
+            const Acc<S> tb = t1[r];
+            d0 = tb.x; d1 = tb.y; d2 = tb.z; d3 = tb.w;
+        }
+        d0 += db * R[0];
+        d1 += db * (R[1] + prm.force);
+        d2 += db * R[2];
+        d3 += db * R[3];
+        if (sa.stage < 3) {
+            b0 = d0; b1 = d1; b2 = d2; b3 = d3;
+        } else {
+            b0 = a0 + d0; b1 = a1 + d1; b2 = a2 + d2; b3 = a3 + d3;
+        }
+    }
+    const double r0 = F32 ? prm.rho0 : 0.0;  // single-precision density is stored as rho - rho0
+    if (sa.stage < 3) {
+        st_out[r] = V4<S>{(S)(f1 - r0), (S)f2, (S)f3, (S)f4};
+        using E = typename std::conditional<F32, float, double>::type;
+        t1[r] = Acc<S>{(E)b0, (E)b1, (E)b2, (E)b3};
+        return 0;
+    }
+    // soln = t1 as primitives (Solver.cpp:431-436); it is the next step's temp
+    const double s0 = b0, s1 = b1 / b0, s2 = b2 / b0, s3 = b3 / b0;
+    st_out[r] = V4<S>{(S)(s0 - r0), (S)s1, (S)s2, (S)s3};
+    t0[r] = double4{b0, b1, b2, b3};
+    if (prm.time_stepping == PLBFS_DT_CONSTANT) dtl[r] = prm.cfl;
+    else dtl[r] = local_dt(prm, visc, s0, s1, m.cfl_area[i], m.vol[i]);
+    sq[0] += R[0] * R[0];
+    sq[1] += R[1] * R[1];
+    sq[2] += R[2] * R[2];
+    sq[3] += R[3] * R[3];
+    if (isnan(f1) || isnan(f2)) return 1;
+    if (f1 / prm.ref_rho > 1000.0) return 2;
+    return 0;
+}
+
+}  // namespace plbfs

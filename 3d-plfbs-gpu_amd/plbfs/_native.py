@@ -32,6 +32,12 @@ EXPORTS = [
     "plbfs_case_string", "plbfs_case_number", "plbfs_case_patches", "plbfs_case_initial_state",
     "plbfs_case_n_objects", "plbfs_case_object_nodes", "plbfs_write_tecplot", "plbfs_write_fields",
     "plbfs_read_fields", "plbfs_program_run",
+    "plbfs_batch_create", "plbfs_batch_destroy", "plbfs_batch_set_state", "plbfs_batch_get_state",
+    "plbfs_batch_set_ibm", "plbfs_batch_run", "plbfs_batch_set_tiling", "plbfs_batch_autotune",
+    "plbfs_batch_layout_info",
+    "plbfs_mpnn_n_params", "plbfs_mpnn_create", "plbfs_mpnn_destroy", "plbfs_mpnn_set_params",
+    "plbfs_mpnn_get_params", "plbfs_mpnn_forward", "plbfs_mpnn_loss_grad", "plbfs_mpnn_adam_step",
+    "plbfs_mpnn_train",
 ]
 
 
@@ -116,6 +122,26 @@ def lib():
                                  ip, ip]
     L.plbfs_mesh_n_face_verts.argtypes = [vp]
     L.plbfs_mesh_n_face_verts.restype = C.c_int64
+    L.plbfs_batch_create.argtypes = [C.POINTER(MeshView), C.POINTER(Params), C.POINTER(Patch), C.c_int32, ip,
+                                     C.c_int32, dp, C.POINTER(vp)]
+    L.plbfs_batch_destroy.argtypes = [vp]
+    L.plbfs_batch_set_state.argtypes = [vp, C.c_int32, dp, dp, dp, dp]
+    L.plbfs_batch_get_state.argtypes = [vp, C.c_int32, dp, dp, dp, dp]
+    L.plbfs_batch_set_ibm.argtypes = [vp, C.c_int32, C.POINTER(IbmNodes)]
+    L.plbfs_batch_run.argtypes = [vp, C.c_int32, C.POINTER(Result)]
+    L.plbfs_batch_set_tiling.argtypes = [vp, C.c_int32, C.c_int32]
+    L.plbfs_batch_autotune.argtypes = [vp, C.c_int32, ip, ip, dp]
+    L.plbfs_batch_layout_info.argtypes = [vp, i64p]
+    fp = C.POINTER(C.c_float)
+    L.plbfs_mpnn_n_params.restype = C.c_int
+    L.plbfs_mpnn_create.argtypes = [C.c_int32, C.c_int32, ip, ip, fp, C.c_int32, C.POINTER(vp)]
+    L.plbfs_mpnn_destroy.argtypes = [vp]
+    L.plbfs_mpnn_set_params.argtypes = [vp, fp]
+    L.plbfs_mpnn_get_params.argtypes = [vp, fp]
+    L.plbfs_mpnn_forward.argtypes = [vp, fp, fp]
+    L.plbfs_mpnn_loss_grad.argtypes = [vp, fp, fp, fp, dp, fp]
+    L.plbfs_mpnn_adam_step.argtypes = [vp, C.c_float, C.c_float, C.c_float, C.c_float]
+    L.plbfs_mpnn_train.argtypes = [vp, C.c_int32, fp, fp, fp, C.c_int32, C.c_float, C.c_uint64, dp]
     _lib = L
     return L
 

@@ -201,3 +201,73 @@ class GpuSolver:
         keys = ["cells", "faces", "items", "stencil_slots", "ghosts", "blocks", "max_items", "gather",
                 "block_cells", "lds_bytes"]
         return dict(zip(keys, (int(x) for x in info)))
+
+
+class GpuBatch:
+    """B cases on one mesh (plbfs_batch_*): per-case viscosity, state and IBM nodes."""
+
+    def __init__(self, hmesh: HostMesh, params: dict, patches: list, case_visc):
+        L = nat.lib()
+        self.mesh = hmesh
+        self.N = hmesh.n_cells
+        self.B = len(case_visc)
+        self._params = make_params(params)
+        self._patches = make_patches(patches)
+        self._bp = np.ascontiguousarray(hmesh.bface_patch(), dtype=np.int32)
+        self._visc = np.ascontiguousarray(case_visc, dtype=np.float64)
+        h = C.c_void_p()
+        nat.check(L.plbfs_batch_create(C.byref(hmesh.view), C.byref(self._params), self._patches, len(patches),
+                                       nat.iptr(self._bp) if self._bp.size else None, self.B, nat.dptr(self._visc),
+                                       C.byref(h)))
+        self.h = h
+        self._ibm_keep = {}
+
+    def close(self):
+        if self.h:
+            nat.lib().plbfs_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_state(self, case, rho, u, v, w):
+        a = [np.ascontiguousarray(x[: self.N], dtype=np.float64) for x in (rho, u, v, w)]
+        nat.check(nat.lib().plbfs_batch_set_state(self.h, int(case), *(nat.dptr(x) for x in a)))
+
+    def state(self, case):
+        a = [np.zeros(self.N) for _ in range(4)]
+        nat.check(nat.lib().plbfs_batch_get_state(self.h, int(case), *(nat.dptr(x) for x in a)))
+        return a
+
+    def set_ibm(self, case, x_ref, x, stiffness, node_area, dt, h, kernel):
+        xr = np.ascontiguousarray(x_ref, dtype=np.float64).reshape(-1)
+        xx = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
+        self._ibm_keep[case] = (xr, xx)
+        nodes = nat.IbmNodes(xr.size // 3, nat.dptr(xr), nat.dptr(xx), stiffness, node_area, dt, h, kernel)
+        nat.check(nat.lib().plbfs_batch_set_ibm(self.h, int(case), C.byref(nodes)))
+
+    def run(self, nsteps):
+        res = (nat.Result * self.B)()
+        rc = nat.lib().plbfs_batch_run(self.h, int(nsteps), res)
+        if rc not in (0, -2, -3, -7):
+            nat.check(rc)
+        return rc, [dict(status=r.status, steps=r.steps, converged=bool(r.converged), residual=list(r.residual),
+                         ms=r.solve_ms) for r in res]
+
+    def set_tiling(self, block_cells, cases_per_block):
+        nat.check(nat.lib().plbfs_batch_set_tiling(self.h, int(block_cells), int(cases_per_block)))
+
+    def autotune(self, trial_steps=5):
+        bc = C.c_int32(0)
+        cp = C.c_int32(0)
+        ms = C.c_double(0)
+        nat.check(nat.lib().plbfs_batch_autotune(self.h, int(trial_steps), C.byref(bc), C.byref(cp), C.byref(ms)))
+        return bc.value, cp.value, ms.value
+
+    def layout_info(self):
+        info = np.zeros(4, dtype=np.int64)
+        nat.check(nat.lib().plbfs_batch_layout_info(self.h, info.ctypes.data_as(C.POINTER(C.c_int64))))
+        return dict(zip(["cases", "block_cells", "cases_per_block", "lds_bytes"], (int(x) for x in info)))

@@ -181,6 +181,42 @@ int plbfs_kernel_stats(plbfs_ctx *ctx, int32_t which, double *total_ms, int32_t 
  * block, gather entries, cells per flux block, LDS bytes per flux block. */
 int plbfs_layout_info(plbfs_ctx *ctx, int64_t *info10);
 
+/* ---- Batched cases ----------------------------------------------------------
+ * B independent cases on one mesh topology, advanced together: the mesh,
+ * layout and LSQ weights are shared; each case has its own viscosity (its
+ * Reynolds number), state, local time steps and immersed nodes.  A case that
+ * converges (or fails) is frozen while the others continue; plbfs_batch_run
+ * returns once every case is frozen or max_steps have passed.  The reference
+ * has no batched solver: each reference run is one call of
+ * gpu_solver::General_Purpose_Solver_mk_i (src/Solver_gpu.h:28-31), and a batch
+ * replaces B such calls on one mesh.  Time stepping local or constant;
+ * fneq_min must be 0 (the face dt then does not depend on the viscosity). */
+typedef struct plbfs_batch plbfs_batch;
+
+int plbfs_batch_create(const plbfs_mesh_view *mesh, const plbfs_params *params, const plbfs_patch *patches,
+                       int32_t n_patches, const int32_t *bface_patch, int32_t n_cases, const double *case_visc,
+                       plbfs_batch **out);
+int plbfs_batch_destroy(plbfs_batch *batch);
+/* State of case case_id (file cell order, [N] each); case_id < 0 sets every
+ * case.  Resets every case's step count and convergence flag. */
+int plbfs_batch_set_state(plbfs_batch *batch, int32_t case_id, const double *rho, const double *u, const double *v,
+                          const double *w);
+int plbfs_batch_get_state(plbfs_batch *batch, int32_t case_id, double *rho, double *u, double *v, double *w);
+/* Immersed nodes of one case; kernel, h and dt must agree across cases. */
+int plbfs_batch_set_ibm(plbfs_batch *batch, int32_t case_id, const plbfs_ibm_nodes *nodes);
+/* results[n_cases]: per-case status, steps, convergence and residual; the
+ * return value is the first failing case's status, or PLBFS_OK. */
+int plbfs_batch_run(plbfs_batch *batch, int32_t max_steps, plbfs_result *results);
+/* Tiling: cells per flux block (multiple of 64, 64..1024) and cases per
+ * workgroup (a divisor of 256). */
+int plbfs_batch_set_tiling(plbfs_batch *batch, int32_t block_cells, int32_t cases_per_block);
+/* Times trial_steps steps of every tiling that fits the LDS, keeps the
+ * fastest, and restores the state and node positions set before the call. */
+int plbfs_batch_autotune(plbfs_batch *batch, int32_t trial_steps, int32_t *block_cells, int32_t *cases_per_block,
+                         double *ms_per_step);
+/* info[0..3]: cases, cells per flux block, cases per workgroup, LDS bytes. */
+int plbfs_batch_layout_info(plbfs_batch *batch, int64_t *info4);
+
 #define PLBFS_KSTAT_GRAD 0
 #define PLBFS_KSTAT_FLUX 1
 #define PLBFS_KSTAT_IBM 2

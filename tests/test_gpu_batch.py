@@ -1,0 +1,167 @@
+"""Batched cases (plbfs_batch_*) against the oracle and the single-case solver.
+
+Each case of a batch runs the same per-cell arithmetic as a single-case
+context, so its fields must equal the single-case GPU run bit for bit, for
+every tiling (cells per flux block x cases per workgroup); in FP64 they must
+also match the oracle run with that case's viscosity to 1e-10 after the same
+steps.  Converged cases freeze while the others continue.
+"""
+import numpy as np
+import pytest
+
+from helpers import channel_mesh, initial_state, oracle_for, perturbed_state, rel_err
+from plbfs import case as pc
+from plbfs import objects
+from plbfs.solver import GpuBatch, GpuSolver, HostMesh
+
+pytestmark = pytest.mark.gpu
+
+RES = [30.0, 60.0, 120.0]
+
+
+def cases_for(res, precision=0, tol=1e-30, check=10):
+    out = []
+    for re in res:
+        c = pc.channel_case(re=re, length_scale=4.0)
+        c.precision = precision
+        c.tolerance = tol
+        c.check_interval = check
+        out.append(c)
+    return out
+
+
+def make_batch(pm, cases):
+    hm = HostMesh.from_polymesh(pm, cases[0].visc, cases[0].gamma, cases[0].fneq_min)
+    bt = GpuBatch(hm, cases[0].params, cases[0].patches_for(pm.patch_names), [c.visc for c in cases])
+    return hm, bt
+
+
+def single_run(pm, case, st, steps, ibm=None):
+    hm = HostMesh.from_polymesh(pm, case.visc, case.gamma, case.fneq_min)
+    s = GpuSolver(hm, case.params, case.patches_for(pm.patch_names))
+    s.set_state(*st)
+    if ibm:
+        s.set_ibm(**ibm)
+    r = s.run(steps)
+    return r, s.state()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_batch_cases_equal_single_runs(gpu_available, precision):
+    pm = channel_mesh(10, 5, 5)
+    cases = cases_for(RES, precision)
+    o = oracle_for(pm, cases[0])
+    st = perturbed_state(cases[0], o, amp=0.01, seed=2)
+    hm, bt = make_batch(pm, cases)
+    bt.set_state(-1, *st)
+    rc, res = bt.run(40)
+    assert rc == 0 and all(r["steps"] == 40 for r in res), res
+    for b, c in enumerate(cases):
+        r1, s1 = single_run(pm, c, st, 40)
+        sb = bt.state(b)
+        for q in range(4):
+            assert np.array_equal(sb[q], s1[q]), (b, q, rel_err(sb[q], s1[q]))
+        assert np.allclose(res[b]["residual"], r1["residual"], rtol=1e-12, atol=0)
+
+
+def test_batch_fp64_matches_oracle_per_case(gpu_available):
+    pm = channel_mesh(10, 5, 5)
+    cases = cases_for(RES)
+    o0 = oracle_for(pm, cases[0])
+    st = perturbed_state(cases[0], o0, amp=0.01, seed=6)
+    hm, bt = make_batch(pm, cases)
+    bt.set_state(-1, *st)
+    assert bt.run(100)[0] == 0
+    for b, c in enumerate(cases):
+        o = oracle_for(pm, c)
+        o.set_state(*st)
+        assert o.run(100)["rc"] == 0
+        so, sb = o.state(), bt.state(b)
+        errs = [rel_err(sb[q], so[q]) for q in range(4)]
+        assert max(errs) < 1e-10, (b, errs)
+
+
+@pytest.mark.parametrize("tiling", [(64, 4), (128, 2), (256, 1), (512, 1), (64, 8)])
+def test_batch_tiling_does_not_change_fields(gpu_available, tiling):
+    pm = channel_mesh(12, 6, 6, tets=True)
+    res = [25.0, 50.0, 75.0, 100.0, 150.0, 200.0, 40.0, 80.0]
+    cases = cases_for(res, precision=1)
+    for c in cases:
+        c.lsq_exact_det = 1
+    o = oracle_for(pm, cases[0])
+    st = perturbed_state(cases[0], o, amp=0.01, seed=8)
+    outs = []
+    for t in [(256, 1), tiling]:
+        hm, bt = make_batch(pm, cases)
+        bt.set_tiling(*t)
+        assert bt.layout_info()["block_cells"] == t[0]
+        bt.set_state(-1, *st)
+        assert bt.run(25)[0] == 0
+        outs.append([bt.state(b) for b in range(len(cases))])
+        bt.close()
+    for b in range(len(cases)):
+        for q in range(4):
+            assert np.array_equal(outs[0][b][q], outs[1][b][q]), (b, q)
+
+
+def test_converged_case_freezes(gpu_available):
+    # the same Re twice, one case started from rest: different convergence steps
+    pm = channel_mesh(12, 6, 6)
+    cases = cases_for([40.0, 40.0], tol=1e-6, check=50)
+    o = oracle_for(pm, cases[0])
+    st0 = initial_state(cases[0], o)
+    st1 = perturbed_state(cases[0], o, amp=0.02, seed=1)
+    hm, bt = make_batch(pm, cases)
+    bt.set_state(0, *st0)
+    bt.set_state(1, *st1)
+    rc, res = bt.run(30000)
+    assert rc == 0 and all(r["converged"] for r in res), res
+    assert res[0]["steps"] != res[1]["steps"]
+    for b, st in enumerate((st0, st1)):
+        r1, s1 = single_run(pm, cases[b], st, 30000)
+        assert r1["converged"] and r1["steps"] == res[b]["steps"], (b, r1["steps"], res[b]["steps"])
+        for q in range(4):
+            assert np.array_equal(bt.state(b)[q], s1[q]), (b, q)
+
+
+def test_batch_per_case_spheres_equal_single_runs(gpu_available):
+    pm = channel_mesh(16, 8, 8)
+    cases = cases_for([40.0, 80.0, 20.0])
+    spheres = [((5.0, 4.0, 4.0), 1.5, 60), ((6.0, 4.2, 3.8), 2.0, 110), ((5.5, 3.9, 4.1), 1.2, 45)]
+    o = oracle_for(pm, cases[0])
+    st = initial_state(cases[0], o)
+    hm, bt = make_batch(pm, cases)
+    bt.set_state(-1, *st)
+    ibms = []
+    for b, (ctr, rad, n) in enumerate(spheres):
+        X, dA = objects.fibonacci_sphere(n, ctr, rad)
+        ibm = dict(x_ref=X, x=X, stiffness=2.0, node_area=dA, dt=cases[b].time_marching_step, h=1.0, kernel=1)
+        bt.set_ibm(b, **ibm)
+        ibms.append(ibm)
+    rc, res = bt.run(30)
+    assert rc == 0, res
+    for b, c in enumerate(cases):
+        r1, s1 = single_run(pm, c, st, 30, ibm=ibms[b])
+        assert r1["rc"] == 0
+        for q in range(4):
+            assert np.array_equal(bt.state(b)[q], s1[q]), (b, q, rel_err(bt.state(b)[q], s1[q]))
+
+
+def test_autotune_keeps_results(gpu_available):
+    pm = channel_mesh(12, 6, 6)
+    cases = cases_for([30.0, 60.0, 90.0, 120.0], precision=1)
+    o = oracle_for(pm, cases[0])
+    st = perturbed_state(cases[0], o, amp=0.01, seed=5)
+    hm, bt = make_batch(pm, cases)
+    bt.set_state(-1, *st)
+    bc, cp, ms = bt.autotune(3)
+    assert bc in (64, 128, 256, 512) and cp in (1, 2, 4) and ms > 0
+    info = bt.layout_info()
+    assert (info["block_cells"], info["cases_per_block"]) == (bc, cp)
+    assert bt.run(20)[0] == 0
+    ref = GpuBatch(hm, cases[0].params, cases[0].patches_for(pm.patch_names), [c.visc for c in cases])
+    ref.set_state(-1, *st)
+    assert ref.run(20)[0] == 0
+    for b in range(4):
+        for q in range(4):
+            assert np.array_equal(bt.state(b)[q], ref.state(b)[q]), (b, q)

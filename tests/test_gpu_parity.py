@@ -92,7 +92,18 @@ def test_channel_100_steps(gpu_available, tets):
     assert max(errs) < 1e-10, errs
 
 
-@pytest.mark.parametrize("block_cells", [512, 768, 1024])
+def test_flux_block_too_large_for_lds_is_a_clean_error(gpu_available, monkeypatch):
+    # 1024 FP64 tets need ~2300 face slots x 32 B > 64 KiB of LDS
+    monkeypatch.setenv("PLBFS_BLOCK_CELLS", "1024")
+    pm = channel_mesh(16, 8, 8, tets=True)
+    case = pc.channel_case(re=40.0, length_scale=6.0)
+    hm = HostMesh.from_polymesh(pm, case.visc, case.gamma, case.fneq_min)
+    with pytest.raises(nat.PlbfsError) as ei:
+        GpuSolver(hm, case.params, case.patches_for(pm.patch_names))
+    assert ei.value.rc == -1 and "LDS" in str(ei.value)
+
+
+@pytest.mark.parametrize("block_cells", [512, 768])
 def test_flux_block_sizes(gpu_available, monkeypatch, block_cells):
     # larger flux blocks change which faces are duplicated on block borders
     # and how many cells one thread updates, not the arithmetic
