@@ -57,7 +57,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
     const int b = (int)(idx - (int64_t)i * a.B);
     if (i >= m.N || a.done[b]) return;
     S g[12];
-    lsq_grad<S>(m, i, st, RecB{a.B, b}, g);
+    lsq_grad<S>(m, i, GlobalStates<S, RecB>{st, RecB{a.B, b}}, g);
     V4<S> *gi = grad + 3 * ((size_t)i * a.B + b);
     gi[0] = V4<S>{g[0], g[1], g[2], g[3]};
     gi[1] = V4<S>{g[4], g[5], g[6], g[7]};
@@ -83,7 +83,8 @@ __global__ __launch_bounds__(kBlock) void kb_flux(DevMesh<S> m, DevParams prm, B
         const int el = k / C, c = k - el * C, b = b0 + c;
         if (b >= B || a.done[b]) continue;
         S F[4];
-        item_flux<S>(m, prm, e0 + el, st_in, grad, RecB{B, b}, a.v3g[b], F);
+        const SideLoader<S, RecB> ld{st_in, grad, m.cgeo, RecB{B, b}, nullptr, nullptr, 0, 0};
+        item_flux<S>(m, prm, e0 + el, ld, a.v3g[b], F);
 #pragma unroll
         for (int q = 0; q < 4; q++) sF[q * stride + el * C + c] = F[q];
     }
@@ -346,12 +347,13 @@ int ibm_upload(plbfs_batch *b) {
     free_list(b->ibm_allocs);
     std::vector<void *> &M = b->ibm_allocs;
     std::vector<double> xr, xx, kda;
-    std::vector<int32_t> ncase, np_off(1, 0), np_cell, cp_cell, cp_case, cp_off(1, 0), cp_node;
+    std::vector<int32_t> ncase, np_off(1, 0), np_cell, cp_cell, cp_case, cp_off(1, 0), cp_node, cp_pair;
     int n_total = 0;
     for (int c = 0; c < b->B; c++) {
         const BatchIbmCase &ic = b->ibm_case[c];
         if (!ic.on) continue;
         const int n = (int)ic.xref.size() / 3;
+        const int32_t pair0 = (int32_t)np_cell.size();
         for (int k = 0; k < n; k++) {
             ncase.push_back(c);
             kda.push_back(ic.kdA);
@@ -361,8 +363,10 @@ int ibm_upload(plbfs_batch *b) {
         for (size_t j = 0; j < ic.pairs.cp_cell.size(); j++) {
             cp_cell.push_back(ic.pairs.cp_cell[j]);
             cp_case.push_back(c);
-            for (int p = ic.pairs.cp_off[j]; p < ic.pairs.cp_off[j + 1]; p++)
+            for (int p = ic.pairs.cp_off[j]; p < ic.pairs.cp_off[j + 1]; p++) {
                 cp_node.push_back(n_total + ic.pairs.cp_node[p]);
+                cp_pair.push_back(pair0 + ic.pairs.cp_pair[p]);
+            }
             cp_off.push_back((int32_t)cp_node.size());
         }
         n_total += n;
@@ -397,14 +401,15 @@ int ibm_upload(plbfs_batch *b) {
     std::vector<double> z((size_t)3 * n_total, 0.0), zn(n_total, 0.0);
     int rc;
     int32_t *nc = nullptr, *npo = nullptr, *npc = nullptr, *cpc = nullptr, *cpk = nullptr, *cpo = nullptr,
-            *cpn = nullptr;
-    double *kd = nullptr, *ccd = nullptr;
+            *cpn = nullptr, *cpp = nullptr;
+    double *kd = nullptr, *ccd = nullptr, *pw = nullptr;
     if ((rc = bupload(M, &a.xref, xr)) || (rc = bupload(M, &a.x, xx)) || (rc = bupload(M, &a.x0, xx)) ||
         (rc = bupload(M, &a.vel, z)) || (rc = bupload(M, &a.force, z)) || (rc = bupload(M, &a.norm, zn)) ||
         (rc = bupload(M, &kd, kda)) || (rc = bupload(M, &nc, ncase)) || (rc = bupload(M, &npo, np_off)) ||
         (rc = bupload(M, &npc, np_cell)) || (rc = bupload(M, &cpc, cp_cell)) || (rc = bupload(M, &cpk, cp_case)) ||
-        (rc = bupload(M, &cpo, cp_off)) || (rc = bupload(M, &cpn, cp_node)) ||
-        (rc = bupload(M, &ccd, b->base->L.centroid)))
+        (rc = bupload(M, &cpo, cp_off)) || (rc = bupload(M, &cpn, cp_node)) || (rc = bupload(M, &cpp, cp_pair)) ||
+        (rc = bupload(M, &ccd, b->base->L.centroid)) ||
+        (rc = balloc(M, (void **)&pw, np_cell.size() * sizeof(double))))
         return rc;
     a.node_kdA = kd;
     a.node_case = nc;
@@ -414,11 +419,17 @@ int ibm_upload(plbfs_batch *b) {
     a.cp_case = cpk;
     a.cp_off = cpo;
     a.cp_node = cpn;
+    a.cp_pair = cpp;
+    a.pw = pw;
     a.cc = ccd;
     a.vol = b->base->dm.vol;
     a.case_done = b->done;
     a.case_status = b->status;
     HIP_TRY(hipMemset(b->fbody, 0, (size_t)3 * b->N * b->B * sizeof(double)));
+    hipLaunchKernelGGL(k_ibm_weights, dim3((n_total + kNodesPerBlock - 1) / kNodesPerBlock), dim3(kIbmThreads), 0,
+                       b->base->stream, a);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(b->base->stream));
     b->ibm_on = true;
     return PLBFS_OK;
 }
@@ -456,7 +467,7 @@ int launch_batch_step(plbfs_batch *b) {
         else
             hipLaunchKernelGGL(k_ibm_nodes<double4>, grid, dim3(kIbmThreads), 0, c->stream, b->ia, b->ctrl,
                                (const double4 *)b->stA);
-        hipLaunchKernelGGL(k_ibm_spread, dim3((b->ia.n_affected + kIbmThreads - 1) / kIbmThreads),
+        hipLaunchKernelGGL(k_ibm_spread, dim3((b->ia.n_affected + kNodesPerBlock - 1) / kNodesPerBlock),
                            dim3(kIbmThreads), 0, c->stream, b->ia, b->ctrl, b->fbody);
     }
     if (b->f32) launch_batch_stages<float>(b, c->dm32);

@@ -38,6 +38,8 @@ struct IbmState {
     double *x = nullptr, *x0 = nullptr, *xref = nullptr, *vel = nullptr, *force = nullptr, *norm = nullptr;
     int32_t *np_off = nullptr, *np_cell = nullptr;           // node -> candidate cells
     int32_t *cp_cell = nullptr, *cp_off = nullptr, *cp_node = nullptr;  // affected cell -> nodes
+    int32_t *cp_pair = nullptr;  // node-major pair index of each cell-major entry
+    double *pw = nullptr;        // [n_pairs] kernel weights at the current node positions
     int n_affected = 0;
     int64_t n_pairs = 0;
     double *cell_c = nullptr;  // [3][N] centroids, Hilbert order
@@ -133,5 +135,6 @@ struct IbmPairs {
     double margin = 0;
     std::vector<int32_t> np_off, np_cell;          // node -> candidate cells (sorted)
     std::vector<int32_t> cp_cell, cp_off, cp_node;  // affected cell -> nodes (node order)
+    std::vector<int32_t> cp_pair;                   // ... and their node-major pair index
 };
 int build_ibm_pairs(const plbfs::Layout &L, const plbfs_ibm_nodes *nodes, IbmPairs &P);

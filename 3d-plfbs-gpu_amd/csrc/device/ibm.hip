@@ -16,6 +16,7 @@
 // Weights are normalised per node, W_cn = delta V_c / sum delta V, which is
 // the reference's formula on its uniform unit grid.  No atomics.
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <unordered_map>
@@ -55,6 +56,8 @@ IbmArgs make_args(plbfs_ctx *c) {
     a.cp_cell = s.cp_cell;
     a.cp_off = s.cp_off;
     a.cp_node = s.cp_node;
+    a.cp_pair = s.cp_pair;
+    a.pw = s.pw;
     a.n_affected = s.n_affected;
     a.cc = s.cell_c;
     a.vol = c->dm.vol;
@@ -73,7 +76,7 @@ int ibm_launch(plbfs_ctx *c) {
     else
         hipLaunchKernelGGL(k_ibm_nodes<double4>, grid, dim3(kIbmThreads), 0, c->stream, a, c->ctrl,
                            (const double4 *)c->stA);
-    hipLaunchKernelGGL(k_ibm_spread, dim3((a.n_affected + kIbmThreads - 1) / kIbmThreads), dim3(kIbmThreads), 0, c->stream, a,
+    hipLaunchKernelGGL(k_ibm_spread, dim3((a.n_affected + kNodesPerBlock - 1) / kNodesPerBlock), dim3(kIbmThreads), 0, c->stream, a,
                        c->ctrl, c->fbody);
     HIP_TRY(hipGetLastError());
     return PLBFS_OK;
@@ -118,7 +121,7 @@ int build_ibm_pairs(const plbfs::Layout &L, const plbfs_ibm_nodes *nodes, IbmPai
     std::vector<int32_t> &np_off = P.np_off, &np_cell = P.np_cell;
     np_off.assign(n + 1, 0);
     np_cell.clear();
-    std::vector<std::pair<int32_t, int32_t>> cell_node;  // (cell, node)
+    std::vector<std::array<int32_t, 3>> cell_node;  // (cell, node, node-major pair)
     for (int k = 0; k < n; k++) {
         const double *X = nodes->x_ref + 3 * k;
         const int64_t bx = (int64_t)std::floor((X[0] - lo[0]) / reach), by = (int64_t)std::floor((X[1] - lo[1]) / reach),
@@ -138,22 +141,24 @@ int build_ibm_pairs(const plbfs::Layout &L, const plbfs_ibm_nodes *nodes, IbmPai
         std::sort(cand.begin(), cand.end());
         if (cand.empty()) return plbfs_fail(PLBFS_ERR_IBM, "node %d has no cell within its kernel support", k);
         for (int32_t i : cand) {
+            cell_node.push_back({i, k, (int32_t)np_cell.size()});
             np_cell.push_back(i);
-            cell_node.push_back({i, k});
         }
         np_off[k + 1] = (int32_t)np_cell.size();
     }
     std::sort(cell_node.begin(), cell_node.end());
-    std::vector<int32_t> &cp_cell = P.cp_cell, &cp_off = P.cp_off, &cp_node = P.cp_node;
+    std::vector<int32_t> &cp_cell = P.cp_cell, &cp_off = P.cp_off, &cp_node = P.cp_node, &cp_pair = P.cp_pair;
     cp_cell.clear();
     cp_node.clear();
+    cp_pair.clear();
     cp_off.assign(1, 0);
     for (size_t p = 0; p < cell_node.size(); p++) {
-        if (p == 0 || cell_node[p].first != cell_node[p - 1].first) {
+        if (p == 0 || cell_node[p][0] != cell_node[p - 1][0]) {
             if (p > 0) cp_off.push_back((int32_t)p);
-            cp_cell.push_back(cell_node[p].first);
+            cp_cell.push_back(cell_node[p][0]);
         }
-        cp_node.push_back(cell_node[p].second);
+        cp_node.push_back(cell_node[p][1]);
+        cp_pair.push_back(cell_node[p][2]);
     }
     cp_off.push_back((int32_t)cell_node.size());
     return PLBFS_OK;
@@ -192,8 +197,16 @@ extern "C" int plbfs_set_ibm(plbfs_ctx *c, const plbfs_ibm_nodes *nodes) {
         (rc = upload(c, &s.vel, z)) || (rc = upload(c, &s.force, z)) || (rc = upload(c, &s.norm, zn)) ||
         (rc = upload(c, &s.np_off, np_off)) || (rc = upload(c, &s.np_cell, np_cell)) ||
         (rc = upload(c, &s.cp_cell, cp_cell)) || (rc = upload(c, &s.cp_off, cp_off)) ||
-        (rc = upload(c, &s.cp_node, cp_node)) || (rc = upload(c, &s.cell_c, c->L.centroid)))
+        (rc = upload(c, &s.cp_node, cp_node)) || (rc = upload(c, &s.cp_pair, P.cp_pair)) ||
+        (rc = upload(c, &s.cell_c, c->L.centroid)) || (rc = dalloc(c, (void **)&s.pw, np_cell.size() * sizeof(double))))
         return rc;
+    {
+        const IbmArgs a = make_args(c);
+        hipLaunchKernelGGL(k_ibm_weights, dim3((n + kNodesPerBlock - 1) / kNodesPerBlock), dim3(kIbmThreads), 0,
+                           c->stream, a);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipStreamSynchronize(c->stream));
+    }
     HIP_TRY(hipMemset(c->fbody, 0, (size_t)3 * N * sizeof(double)));
     s.on = true;
     c->prm.use_body_force = 1;

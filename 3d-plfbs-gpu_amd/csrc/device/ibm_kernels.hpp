@@ -6,8 +6,8 @@
 //                 from the step-start state (src/immersed_boundary_method.cu:13-90),
 //                 move X = X0 + U dt (:244-295), spring force F = -k dA (X - X_ref)
 //                 (:161-190), and the spreading normaliser at the new position;
-//   k_ibm_spread  one thread per affected (cell, case) record: sum of the
-//                 node forces over its pairs in node order (:299-375).
+//   k_ibm_spread  one wavefront per affected (cell, case) record: sum of the
+//                 node forces over its pairs (:299-375).
 // Weights are normalised per node, W_cn = delta V_c / sum delta V, which is the
 // reference's formula on its uniform unit grid.  No atomics.
 //
@@ -54,7 +54,9 @@ struct IbmArgs {
     const int32_t *node_case;  // per-node case, or null (case 0)
     const int32_t *np_off, *np_cell;
     const int32_t *cp_cell, *cp_off, *cp_node;
+    const int32_t *cp_pair;  // node-major pair index of every cell-major entry
     const int32_t *cp_case;  // per affected record case, or null (case 0)
+    double *pw;              // [n_pairs] kernel weight delta(x_c - X_k) at the current node position
     const int32_t *case_done;  // per-case freeze flags, or null
     int *case_status;          // per-case status, or null
     int n_affected;
@@ -74,9 +76,10 @@ __global__ __launch_bounds__(kIbmThreads) void k_ibm_nodes(IbmArgs a, Ctrl *__re
     const double X[3] = {a.x[k], a.x[(size_t)n + k], a.x[2 * (size_t)n + k]};
     const int p0 = a.np_off[k], p1 = a.np_off[k + 1];
     double S = 0, su = 0, sv = 0, sw = 0;
+    // weights at X: computed at set-up or by the previous step's second pass
     for (int p = p0 + lane; p < p1; p += 64) {
         const int c = a.np_cell[p];
-        const double d = ibm_delta(a.kernel, a.ih, a.cc, a.N, c, X);
+        const double d = a.pw[p];
         if (d != 0.0) {
             const double wv = d * a.vol[c];
             const SV s = st[(size_t)c * a.B + b];
@@ -104,6 +107,7 @@ __global__ __launch_bounds__(kIbmThreads) void k_ibm_nodes(IbmArgs a, Ctrl *__re
     for (int p = p0 + lane; p < p1; p += 64) {
         const int c = a.np_cell[p];
         const double d = ibm_delta(a.kernel, a.ih, a.cc, a.N, c, Xn);
+        a.pw[p] = d;
         if (d != 0.0) S2 += d * a.vol[c];
     }
     S2 = wave_allsum(S2);
@@ -128,31 +132,49 @@ __global__ __launch_bounds__(kIbmThreads) void k_ibm_nodes(IbmArgs a, Ctrl *__re
     }
 }
 
-// fbody is [3][B*N] component-major over records.
+// Kernel weights of every pair at the nodes' current positions (set-up).
+__global__ __launch_bounds__(kIbmThreads) void k_ibm_weights(IbmArgs a) {
+    const int lane = threadIdx.x & 63;
+    const int k = blockIdx.x * kNodesPerBlock + (threadIdx.x >> 6);
+    if (k >= a.n) return;
+    const int n = a.n;
+    const double X[3] = {a.x[k], a.x[(size_t)n + k], a.x[2 * (size_t)n + k]};
+    for (int p = a.np_off[k] + lane; p < a.np_off[k + 1]; p += 64)
+        a.pw[p] = ibm_delta(a.kernel, a.ih, a.cc, a.N, a.np_cell[p], X);
+}
+
+// One 64-wide wavefront per affected record; lanes stride over its pairs in
+// node order and a fixed butterfly sums them.  fbody is [3][B*N]
+// component-major over records.
 __global__ __launch_bounds__(kIbmThreads) void k_ibm_spread(IbmArgs a, const Ctrl *__restrict__ ctrl,
                                                             double *__restrict__ fbody) {
     if (ctrl->done) return;
-    const int j = blockIdx.x * kIbmThreads + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int j = blockIdx.x * kNodesPerBlock + (threadIdx.x >> 6);
     if (j >= a.n_affected) return;
     const int c = a.cp_cell[j];
     const int b = a.cp_case ? a.cp_case[j] : 0;
     if (a.case_done && a.case_done[b]) return;
     const int n = a.n;
-    double f[3] = {0, 0, 0};
-    for (int p = a.cp_off[j]; p < a.cp_off[j + 1]; p++) {
+    double f0 = 0, f1 = 0, f2 = 0;
+    for (int p = a.cp_off[j] + lane; p < a.cp_off[j + 1]; p += 64) {
         const int k = a.cp_node[p];
-        const double X[3] = {a.x[k], a.x[(size_t)n + k], a.x[2 * (size_t)n + k]};
-        const double d = ibm_delta(a.kernel, a.ih, a.cc, a.N, c, X);
+        const double d = a.pw[a.cp_pair[p]];  // at the node's new position
         if (d == 0.0) continue;
         const double q = d / a.norm[k];
-        f[0] += a.force[k] * q;
-        f[1] += a.force[(size_t)n + k] * q;
-        f[2] += a.force[2 * (size_t)n + k] * q;
+        f0 += a.force[k] * q;
+        f1 += a.force[(size_t)n + k] * q;
+        f2 += a.force[2 * (size_t)n + k] * q;
     }
-    const size_t NR = (size_t)a.N * a.B, r = (size_t)c * a.B + b;
-    fbody[r] = f[0];
-    fbody[NR + r] = f[1];
-    fbody[2 * NR + r] = f[2];
+    f0 = wave_allsum(f0);
+    f1 = wave_allsum(f1);
+    f2 = wave_allsum(f2);
+    if (lane == 0) {
+        const size_t NR = (size_t)a.N * a.B, r = (size_t)c * a.B + b;
+        fbody[r] = f0;
+        fbody[NR + r] = f1;
+        fbody[2 * NR + r] = f2;
+    }
 }
 
 }  // namespace

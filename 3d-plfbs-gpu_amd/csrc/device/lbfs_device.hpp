@@ -77,7 +77,8 @@ template <typename S>
 struct DevMesh {
     int N, G, n_blocks, max_items;
     int xcd_remap;
-    int block_cells;  // cells per flux block (a multiple of the 256-thread block)
+    int block_cells;  // cells per flux block (a multiple of 64)
+    int stage;        // 1: flux blocks stage their own cells' records in LDS
     int64_t E;
     const uint8_t *st_cnt;
     const int32_t *st_nbr;

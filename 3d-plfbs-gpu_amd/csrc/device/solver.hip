@@ -38,16 +38,74 @@ constexpr int kFinalizeThreads = 256;
 // Kernels
 // ---------------------------------------------------------------------------
 
+// Byte offset of the staged records behind the face slots in a flux block's LDS.
+template <typename S>
+__host__ __device__ inline size_t staged_offset(int max_items) {
+    return ((size_t)4 * max_items * sizeof(S) + 31) / 32 * 32;
+}
+
+// Dynamic LDS of one flux block: face slots, and with staging the block's own
+// cell records (+ centroid records in FP64 storage).
+template <typename S>
+inline size_t flux_lds_bytes(int max_items, int block_cells, bool stage) {
+    size_t b = staged_offset<S>(max_items);
+    if (stage) {
+        b += (size_t)block_cells * 4 * sizeof(V4<S>);
+        if (std::is_same<S, double>::value) b += (size_t)block_cells * sizeof(double4);
+    }
+    return b;
+}
+
+// Face-item record loader of flux block blk; with staging, first copies the
+// block's own cell records into LDS behind the face slots (then barriers).
+template <typename S>
+__device__ __forceinline__ SideLoader<S, Rec1> block_loader(const DevMesh<S> &m, const V4<S> *st, const V4<S> *grad,
+                                                             double *lds, int blk) {
+    SideLoader<S, Rec1> ld;
+    ld.st = st;
+    ld.grad = grad;
+    ld.cgeo = m.cgeo;
+    ld.c0 = blk * m.block_cells;
+    ld.nc = 0;
+    ld.sRec = nullptr;
+    ld.sCg = nullptr;
+    if (m.stage) {
+        char *base = reinterpret_cast<char *>(lds) + staged_offset<S>(m.max_items);
+        ld.nc = min(m.N, ld.c0 + m.block_cells) - ld.c0;
+        V4<S> *sRec = reinterpret_cast<V4<S> *>(base);
+        double4 *sCg = std::is_same<S, double>::value
+                           ? reinterpret_cast<double4 *>(base + (size_t)m.block_cells * 4 * sizeof(V4<S>))
+                           : nullptr;
+        stage_records<S>(st, grad, std::is_same<S, double>::value ? m.cgeo : nullptr, Rec1{}, ld.c0, ld.nc, sRec,
+                         sCg);
+        ld.sRec = sRec;
+        ld.sCg = sCg;
+        __syncthreads();
+    }
+    return ld;
+}
+
 // Occupancy 8 waves/SIMD (<= 64 VGPRs) keeps enough gathers in flight; the
 // compiler otherwise settles at 92 VGPRs (5 waves) for the FP64 instance.
 template <typename S>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) void k_grad(DevMesh<S> m, const Ctrl *__restrict__ ctrl,
                                                  const V4<S> *__restrict__ st, V4<S> *__restrict__ grad) {
     if (ctrl->done) return;
-    const int i = xcd_block(blockIdx.x, gridDim.x, m.xcd_remap) * kBlock + threadIdx.x;
-    if (i >= m.N) return;
+    const int c0 = xcd_block(blockIdx.x, gridDim.x, m.xcd_remap) * kBlock;
+    const int i = c0 + threadIdx.x;
     S g[12];
-    lsq_grad<S>(m, i, st, Rec1{}, g);
+    if (m.stage) {
+        // the block's own states in LDS: most stencil neighbours are in the block
+        __shared__ V4<S> sSt[kBlock];
+        const int nc = min(m.N - c0, kBlock);
+        if ((int)threadIdx.x < nc) sSt[threadIdx.x] = st[i];
+        __syncthreads();
+        if (i >= m.N) return;
+        lsq_grad<S>(m, i, StagedStates<S>{st, sSt, c0, nc}, g);
+    } else {
+        if (i >= m.N) return;
+        lsq_grad<S>(m, i, GlobalStates<S, Rec1>{st, Rec1{}}, g);
+    }
     V4<S> *gi = grad + (size_t)3 * i;
     gi[0] = V4<S>{g[0], g[1], g[2], g[3]};
     gi[1] = V4<S>{g[4], g[5], g[6], g[7]};
@@ -69,9 +127,10 @@ __global__ __launch_bounds__(kBlock) void k_flux_update(DevMesh<S> m, DevParams 
     const int e0 = m.blk_item_off[blk], e1 = m.blk_item_off[blk + 1];
     const int nslot = m.max_items;
     S *sF = reinterpret_cast<S *>(lds);
+    const SideLoader<S, Rec1> ld = block_loader<S>(m, st_in, grad, lds, blk);
     for (int e = e0 + threadIdx.x; e < e1; e += kBlock) {
         S F[4];
-        item_flux<S>(m, prm, e, st_in, grad, Rec1{}, prm.visc3_invgam, F);
+        item_flux<S>(m, prm, e, ld, prm.visc3_invgam, F);
         const int sl = e - e0;
 #pragma unroll
         for (int q = 0; q < 4; q++) sF[q * nslot + sl] = F[q];
@@ -224,9 +283,10 @@ __global__ __launch_bounds__(kBlock) void k_residual_only(DevMesh<S> m, DevParam
     const int e0 = m.blk_item_off[blk], e1 = m.blk_item_off[blk + 1];
     const int nslot = m.max_items;
     S *sF = reinterpret_cast<S *>(lds);
+    const SideLoader<S, Rec1> ld = block_loader<S>(m, st_in, grad, lds, blk);
     for (int e = e0 + threadIdx.x; e < e1; e += kBlock) {
         S F[4];
-        item_flux<S>(m, prm, e, st_in, grad, Rec1{}, prm.visc3_invgam, F);
+        item_flux<S>(m, prm, e, ld, prm.visc3_invgam, F);
 #pragma unroll
         for (int q = 0; q < 4; q++) sF[q * nslot + (e - e0)] = F[q];
     }
@@ -270,14 +330,21 @@ extern "C" int plbfs_device_count(void) {
     return n;
 }
 
+static bool stage_default() {
+    const char *sg = std::getenv("PLBFS_STAGE");
+    return !(sg && sg[0] == '0');
+}
+
 extern "C" int plbfs_create(const plbfs_mesh_view *mesh, const plbfs_params *params, const plbfs_patch *patches,
                             int32_t n_patches, const int32_t *bface_patch, plbfs_ctx **out) {
-    // cells per flux block: a multiple of the 256-thread block; larger blocks
-    // duplicate fewer border faces but need more LDS
+    // cells per flux block (a multiple of 64): larger blocks duplicate fewer
+    // border faces but need more LDS; with staged records FP64 blocks hold 192
+    // cells to stay within the 64 KiB of one workgroup
     int block_cells = kFluxBlockCells;
+    if (params && params->precision == PLBFS_FP64 && stage_default()) block_cells = 192;
     if (const char *bc = std::getenv("PLBFS_BLOCK_CELLS")) block_cells = std::atoi(bc);
-    if (block_cells < kBlock || block_cells % kBlock || block_cells > 4 * kBlock)
-        return plbfs_fail(PLBFS_ERR_INPUT, "PLBFS_BLOCK_CELLS must be a multiple of %d up to %d", kBlock, 4 * kBlock);
+    if (block_cells < 64 || block_cells % 64 || block_cells > 4 * kBlock)
+        return plbfs_fail(PLBFS_ERR_INPUT, "PLBFS_BLOCK_CELLS must be a multiple of 64 up to %d", 4 * kBlock);
     return create_ctx(mesh, params, patches, n_patches, bface_patch, block_cells, 1, out);
 }
 
@@ -304,7 +371,12 @@ int create_ctx(const plbfs_mesh_view *mesh, const plbfs_params *params, const pl
     }
     const Layout &L = c->L;
     const size_t sbytes = c->f32 ? sizeof(float) : sizeof(double);
-    c->lds_bytes = (size_t)L.max_items_per_block * 4 * sbytes * lds_cases;
+    // staging the block's own cell records in LDS (single-case blocks)
+    const bool stage = lds_cases == 1 && stage_default();
+    c->lds_bytes = lds_cases == 1
+                       ? (c->f32 ? flux_lds_bytes<float>(L.max_items_per_block, block_cells, stage)
+                                 : flux_lds_bytes<double>(L.max_items_per_block, block_cells, stage))
+                       : (size_t)L.max_items_per_block * 4 * sbytes * lds_cases;
     if (c->lds_bytes > kMaxLds) {
         delete c;
         return plbfs_fail(PLBFS_ERR_INPUT, "a block needs %zu B of LDS for %d face slots x %d cases (limit %d B)",
@@ -326,6 +398,7 @@ int create_ctx(const plbfs_mesh_view *mesh, const plbfs_params *params, const pl
     d.max_items = L.max_items_per_block;
     d.E = L.n_items;
     d.block_cells = L.block_cells;
+    d.stage = stage ? 1 : 0;
     d.g0x = L.ghost0[0];
     d.g0y = L.ghost0[1];
     d.g0z = L.ghost0[2];
@@ -378,6 +451,7 @@ int create_ctx(const plbfs_mesh_view *mesh, const plbfs_params *params, const pl
         e.N = d.N; e.G = d.G; e.n_blocks = d.n_blocks; e.max_items = d.max_items; e.E = d.E;
         e.xcd_remap = d.xcd_remap;
         e.block_cells = d.block_cells;
+        e.stage = d.stage;
         e.it_fc = nullptr;
         e.cgeo = nullptr;
         e.g0x = d.g0x; e.g0y = d.g0y; e.g0z = d.g0z;

@@ -61,9 +61,29 @@ __device__ __forceinline__ T wave_sum(T x) {
 // Least-squares gradient of cell i (src/gradients.cpp:319-389): folded stencil
 // weights times neighbour differences; ghost neighbours from the owner's patch.
 template <typename S, typename R>
-__device__ __forceinline__ void lsq_grad(const DevMesh<S> &m, int i, const V4<S> *__restrict__ st, R rec, S g[12]) {
+struct GlobalStates {
+    const V4<S> *__restrict__ st;
+    R rec;
+    __device__ __forceinline__ V4<S> operator()(int j) const { return st[rec(j)]; }
+};
+
+// States of cells [c0, c0 + nc) from an LDS copy, the rest from global memory.
+template <typename S>
+struct StagedStates {
+    const V4<S> *__restrict__ st;
+    const V4<S> *sSt;
+    int c0, nc;
+    __device__ __forceinline__ V4<S> operator()(int j) const {
+        const unsigned rel = (unsigned)(j - c0);
+        if (rel < (unsigned)nc) return sSt[rel];
+        return st[j];
+    }
+};
+
+template <typename S, typename ST>
+__device__ __forceinline__ void lsq_grad(const DevMesh<S> &m, int i, const ST &state, S g[12]) {
     const int N = m.N;
-    const V4<S> q0 = st[rec(i)];
+    const V4<S> q0 = state(i);
     const S p0[4] = {q0.x, q0.y, q0.z, q0.w};
 #pragma unroll
     for (int q = 0; q < 12; q++) g[q] = (S)0;
@@ -74,7 +94,7 @@ __device__ __forceinline__ void lsq_grad(const DevMesh<S> &m, int i, const V4<S>
             const int j = m.st_nbr[(size_t)s * N + i];
             S pj[4];
             if (j >= 0) {
-                const V4<S> qj = st[rec(j)];
+                const V4<S> qj = state(j);
                 pj[0] = qj.x;
                 pj[1] = qj.y;
                 pj[2] = qj.z;
@@ -98,25 +118,67 @@ __device__ __forceinline__ void lsq_grad(const DevMesh<S> &m, int i, const V4<S>
 }
 
 template <typename S>
-__device__ __forceinline__ void load_side(const V4<S> *__restrict__ st, const V4<S> *__restrict__ grad, size_t r,
-                                          S p[4], S g[12]) {
-    const V4<S> q = st[r];
+__device__ __forceinline__ void unpack_side(V4<S> q, V4<S> a, V4<S> b, V4<S> d, S p[4], S g[12]) {
     p[0] = q.x;
     p[1] = q.y;
     p[2] = q.z;
     p[3] = q.w;
-    const V4<S> *gc = grad + 3 * r;
-    const V4<S> a = gc[0], b = gc[1], d = gc[2];
     g[0] = a.x; g[1] = a.y; g[2] = a.z; g[3] = a.w;
     g[4] = b.x; g[5] = b.y; g[6] = b.z; g[7] = b.w;
     g[8] = d.x; g[9] = d.y; g[10] = d.z; g[11] = d.w;
 }
 
+// Where a face item reads a cell's state + gradient record (and, in FP64
+// storage, its centroid): cells of the block's own range [c0, c0 + nc) from
+// the copy the block staged in LDS (sRec: state, grad x3 per cell; sCg:
+// centroid + 1/V), every other cell from global memory.  With nc = 0 every
+// read goes to global memory.
+template <typename S, typename R>
+struct SideLoader {
+    const V4<S> *__restrict__ st;
+    const V4<S> *__restrict__ grad;
+    const double4 *__restrict__ cgeo;
+    R rec;
+    const V4<S> *sRec;
+    const double4 *sCg;
+    int c0, nc;
+    __device__ __forceinline__ void side(int c, S p[4], S g[12]) const {
+        const unsigned rel = (unsigned)(c - c0);
+        if (rel < (unsigned)nc) {
+            const V4<S> *q = sRec + 4 * rel;
+            unpack_side<S>(q[0], q[1], q[2], q[3], p, g);
+        } else {
+            const size_t r = rec(c);
+            const V4<S> *gc = grad + 3 * r;
+            unpack_side<S>(st[r], gc[0], gc[1], gc[2], p, g);
+        }
+    }
+    __device__ __forceinline__ double4 centroid(int c) const {
+        const unsigned rel = (unsigned)(c - c0);
+        if (rel < (unsigned)nc) return sCg[rel];
+        return cgeo[c];
+    }
+};
+
+// Copies the records of cells [c0, c0 + nc) of one case into LDS (4 V4 per
+// cell) and, when cgeo is given, their centroid records.
+template <typename S, typename R>
+__device__ __forceinline__ void stage_records(const V4<S> *__restrict__ st, const V4<S> *__restrict__ grad,
+                                              const double4 *__restrict__ cgeo, R rec, int c0, int nc,
+                                              V4<S> *sRec, double4 *sCg) {
+    for (int k = threadIdx.x; k < nc * 4; k += blockDim.x) {
+        const int r = k >> 2, q = k & 3;
+        const size_t g = rec(c0 + r);
+        sRec[k] = q == 0 ? st[g] : grad[3 * g + q - 1];
+    }
+    if (cgeo)
+        for (int k = threadIdx.x; k < nc; k += blockDim.x) sCg[k] = cgeo[c0 + k];
+}
+
 // Flux of face work item e (src/Solver.cpp:286-347 with populate_lattice_macros
 // :840-1274); v3g = 3 nu / gamma of the case.
-template <typename S, typename R>
-__device__ __forceinline__ void item_flux(const DevMesh<S> &m, const DevParams &prm, int64_t e,
-                                          const V4<S> *__restrict__ st, const V4<S> *__restrict__ grad, R rec,
+template <typename S, typename LD>
+__device__ __forceinline__ void item_flux(const DevMesh<S> &m, const DevParams &prm, int64_t e, const LD &ld,
                                           double v3g, S F[4]) {
     const int64_t E = m.E;
     const int o = m.it_own[e];
@@ -126,12 +188,12 @@ __device__ __forceinline__ void item_flux(const DevMesh<S> &m, const DevParams &
 #pragma unroll
         for (int q = 0; q < 5; q++) geo[q] = m.it_geo[(size_t)q * E + e];
         const double fx = m.it_fc[e], fy = m.it_fc[(size_t)E + e], fz = m.it_fc[2 * (size_t)E + e];
-        const double4 co = m.cgeo[o];
+        const double4 co = ld.centroid(o);
         geo[5] = co.x - fx;
         geo[6] = co.y - fy;
         geo[7] = co.z - fz;
         if (nb >= 0) {
-            const double4 cn = m.cgeo[nb];
+            const double4 cn = ld.centroid(nb);
             geo[8] = cn.x - fx;
             geo[9] = cn.y - fy;
             geo[10] = cn.z - fz;
@@ -147,14 +209,14 @@ __device__ __forceinline__ void item_flux(const DevMesh<S> &m, const DevParams &
     const S n[3] = {geo[0], geo[1], geo[2]};
     const S A = geo[3], dt = geo[4];
     S po[4], go[12];
-    load_side<S>(st, grad, rec(o), po, go);
+    ld.side(o, po, go);
     S pI[4], pN[4], p0[4], gN[12];
     if (nb >= 0 || nb == -1) {
         // interior branch: a real neighbour, or the ghost whose index equals
         // n_cells (the reference's strict "neighbour > n_cells" test, Solver.cpp:861)
         S pn[4];
         if (nb >= 0) {
-            load_side<S>(st, grad, rec(nb), pn, gN);
+            ld.side(nb, pn, gN);
         } else {
             const DevPatch P = m.patches[m.b_patch[0]];
             ghost_state<S>(P, po, pn);

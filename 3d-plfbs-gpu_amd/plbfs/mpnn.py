@@ -134,6 +134,32 @@ class Mpnn:
         return hist
 
 
+H = 64
+N_LAYERS = 6
+
+
+def mlp_shapes():
+    """(name, k_in, k_out) of every MLP in the library's parameter order."""
+    out = [("enc", D, H)]
+    for l in range(N_LAYERS):
+        out.append(("msg%d" % l, 2 * H + D, H))
+        out.append(("upd%d" % l, 2 * H, H))
+    out.append(("dec", H, D))
+    return out
+
+
+def init_params(seed=0):
+    """He-normal first layers, 1/sqrt(64) second layers (halved in the residual
+    update MLPs), zero biases; flat FP32 vector."""
+    rng = np.random.default_rng(seed)
+    P = {}
+    for name, k, o in mlp_shapes():
+        P[name] = dict(W1=rng.normal(0, np.sqrt(2.0 / k), (k, H)), b1=np.zeros(H),
+                       W2=rng.normal(0, np.sqrt(1.0 / H), (H, o)) * (0.5 if name.startswith("upd") else 1.0),
+                       b2=np.zeros(o))
+    return flatten_params(P, mlp_shapes())
+
+
 def flatten_params(P, shapes):
     """Flat FP32 vector in the library's order from {name: dict(W1, b1, W2, b2)}."""
     out = []

@@ -99,6 +99,101 @@ def cpu_baseline(pm, case, st, seconds_hint):
                        "(restatement of src/Solver.cpp), %.1f s" % (pm.n_cells, dt))
 
 
+def run_batch(args):
+    """B = 128 cases (seeded Re, sphere size and position) on one jittered
+    64 x 32 x 32 hex channel, FP32 storage, autotuned tiling; value =
+    case-cell-updates/s."""
+    from plbfs import warmstart as ws
+    from plbfs.solver import GpuBatch
+    B = args.cases
+    pm = meshgen.box_mesh(64, 32, 32, lengths=(64.0, 32.0, 32.0), jitter=0.15, jitter_seed=12, permute_seed=13,
+                          patch_types=pc.CHANNEL_PATCHES)
+    hm = HostMesh.from_polymesh(pm, 0.01, 1.0, 0.0)
+    draws = ws.sample_cases(B, (64.0, 32.0, 32.0), seed=2024)
+    cases = [ws.case_for(d, precision=1 if args.precision == "fp32" else 0, tol=0.0, check=max(args.steps, 1))
+             for d in draws]
+    bt = GpuBatch(hm, cases[0].params, cases[0].patches_for(pm.patch_names), [c.visc for c in cases])
+    geo = hm.geometry()
+    bt.set_state(-1, *ws.uniform_state(cases[0], geo, pm.n_cells))
+    nodes = 0
+    for b, (d, c) in enumerate(zip(draws, cases)):
+        bt.set_ibm(b, **ws.ibm_args(d, c))
+        nodes += d.n_nodes
+    t = time.perf_counter()
+    bc, cp, tune_ms = bt.autotune(3)
+    print("autotune: block_cells %d cases_per_block %d (%.3f ms/step, %.1f s)" % (bc, cp, tune_ms,
+          time.perf_counter() - t), file=sys.stderr, flush=True)
+    if args.warmup > 0:
+        bt.run(args.warmup)
+    rc, res = bt.run(args.steps)
+    assert rc == 0 and all(r["steps"] == args.warmup + args.steps for r in res), (rc, res[:2])
+    ms = res[0]["ms"]
+    value = pm.n_cells * B * args.steps / (ms * 1e-3)
+    P = 4 if args.precision == "fp32" else 8
+    # contract figure per case-cell-update: 4 stages x (48 P per case + geometry shared by the B cases)
+    # (stencil slots = cell-face incidences F + NI; geometry stored at the field precision)
+    F, N, NI = pm.n_faces, pm.n_cells, pm.n_internal
+    geo_stage = (F + NI) * (3 * P + 4) + F * (8 + 10 * P) + N * 3 * P
+    b_update = 4 * (48 * P + geo_stage / (N * B))
+    step_gbs = value * b_update / 1e9
+    out = {
+        "metric": METRIC, "value": value, "unit": "case-cell-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32" if P == 4 else "f64",
+        "data": "synthetic mesh (seeded), seeded case draws, uniform-inflow initial state",
+        "config": {"workload": "%d cases on a jittered 64x32x32 hex channel (65,536 cells), Re~U[20,200], "
+                               "sphere D~U[6,14] cells (%d Lagrangian nodes in total), Peskin-4" % (B, nodes),
+                   "cases": B, "cells": N, "faces": F, "block_cells": bc, "cases_per_block": cp},
+        "roofline": {"bound": "hbm", "achieved": step_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": step_gbs / HBM_PEAK_GBS, "traffic": None, "kernel": "whole step (graph replay)",
+                     "alg_bytes_per_case_cell_update": b_update},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out))
+
+
+def run_to_tolerance(args):
+    """Wall time to max(u, v, w) residual 1e-6 from the uniform start, plain
+    FP64 and FP32 storage, on the chosen workload; velocity rel-L2 of FP32
+    against FP64 at convergence.  One JSON line."""
+    pm, case, desc, ibm = workload(args.workload)
+    if args.no_ibm:
+        ibm = None
+    hm = HostMesh.from_polymesh(pm, case.visc, case.gamma, case.fneq_min)
+    cells = hm.geometry()
+    st = case.initial_state(cells["cx"], cells["cy"], cells["cz"])
+    out = {}
+    states = {}
+    for prec in ("fp64", "fp32"):
+        case.tolerance = args.tol
+        case.check_interval = 100
+        case.precision = 1 if prec == "fp32" else 0
+        s = GpuSolver(hm, case.params, case.patches_for(pm.patch_names))
+        s.set_state(*st)
+        if ibm:
+            X, dA = objects.fibonacci_sphere(ibm["n_nodes"], ibm["centre"], ibm["radius"])
+            s.set_ibm(X, X, stiffness=ibm["stiffness"], node_area=dA, dt=case.time_marching_step, h=ibm["h"],
+                      kernel=ibm["kernel"])
+        t = time.perf_counter()
+        r = s.run(args.max_steps)
+        wall = time.perf_counter() - t
+        h = r["hist"]
+        marks = {str(k): float(max(h[k - 1, 1:])) for k in (1, 100, 1000, 5000, 10000, 20000, 40000, 80000)
+                 if k <= len(h)}
+        out[prec] = dict(steps=r["steps"], converged=r["converged"], rc=r["rc"], seconds=wall,
+                         device_ms=r["ms"], residual=max(r["residual"][1:]), history=marks)
+        print("%s: %d steps, converged %s, %.1f s" % (prec, r["steps"], r["converged"], wall), file=sys.stderr,
+              flush=True)
+        states[prec] = s.state()
+        s.close()
+    u64 = np.stack(states["fp64"][1:], 1)
+    u32 = np.stack(states["fp32"][1:], 1)
+    out["fp32_velocity_rel_l2_vs_fp64"] = float(np.linalg.norm(u32 - u64) / np.linalg.norm(u64))
+    out["workload"] = desc
+    out["tolerance"] = args.tol
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -109,6 +204,11 @@ def main():
     ap.add_argument("--no-ibm", action="store_true", help="leave the immersed sphere out")
     ap.add_argument("--precision", choices=("fp64", "fp32"), default="fp64",
                     help="storage precision of fields, gradients and face data")
+    ap.add_argument("--cases", type=int, default=128, help="cases of the batch128 workload")
+    ap.add_argument("--time-to-tol", action="store_true",
+                    help="measure wall time to the residual tolerance (FP64 and FP32) instead of throughput")
+    ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--max-steps", type=int, default=60000)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a device copy of 1 GiB first (PMC FETCH/WRITE calibration)")
     args = ap.parse_args()
@@ -127,6 +227,10 @@ def main():
         torch.cuda.synchronize()
         print("copy calibration: 1073741824 B read + written", file=sys.stderr, flush=True)
 
+    if args.workload == "batch128":
+        return run_batch(args)
+    if args.time_to_tol:
+        return run_to_tolerance(args)
     t0 = time.perf_counter()
     pm, case, desc, ibm = workload(args.workload)
     if args.no_ibm:

@@ -40,6 +40,24 @@ def test_fp32_residual_close_to_oracle(gpu_available, tets):
         assert rel_err(g[:, q], go[: pm.n_cells, q]) < 2e-5, q
 
 
+@pytest.mark.parametrize("stage", [0, 1])
+def test_fp32_staging_is_bitwise_neutral(gpu_available, monkeypatch, stage):
+    pm = channel_mesh(12, 6, 6, tets=True)
+    case = pc.channel_case(re=40.0, length_scale=4.0)
+    case.lsq_exact_det = 1
+    o = oracle_for(pm, case)
+    st = perturbed_state(case, o, amp=0.01, seed=12)
+    outs = []
+    for stg in (stage, 1 - stage):
+        monkeypatch.setenv("PLBFS_STAGE", str(stg))
+        s = solver(pm, case, 1)
+        s.set_state(*st)
+        assert s.run(30)["rc"] == 0
+        outs.append(s.state())
+    for q in range(4):
+        assert np.array_equal(outs[0][q], outs[1][q]), q
+
+
 @pytest.mark.parametrize("tets", [False, True])
 def test_fp32_100_steps_close_to_oracle(gpu_available, tets):
     pm = channel_mesh(10, 5, 5, tets=tets)

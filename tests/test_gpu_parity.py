@@ -95,6 +95,7 @@ def test_channel_100_steps(gpu_available, tets):
 def test_flux_block_too_large_for_lds_is_a_clean_error(gpu_available, monkeypatch):
     # 1024 FP64 tets need ~2300 face slots x 32 B > 64 KiB of LDS
     monkeypatch.setenv("PLBFS_BLOCK_CELLS", "1024")
+    monkeypatch.setenv("PLBFS_STAGE", "0")
     pm = channel_mesh(16, 8, 8, tets=True)
     case = pc.channel_case(re=40.0, length_scale=6.0)
     hm = HostMesh.from_polymesh(pm, case.visc, case.gamma, case.fneq_min)
@@ -103,11 +104,14 @@ def test_flux_block_too_large_for_lds_is_a_clean_error(gpu_available, monkeypatc
     assert ei.value.rc == -1 and "LDS" in str(ei.value)
 
 
-@pytest.mark.parametrize("block_cells", [512, 768])
-def test_flux_block_sizes(gpu_available, monkeypatch, block_cells):
-    # larger flux blocks change which faces are duplicated on block borders
-    # and how many cells one thread updates, not the arithmetic
+@pytest.mark.parametrize("block_cells,stage", [(64, 1), (128, 1), (192, 0), (256, 0), (512, 0), (768, 0)])
+def test_flux_block_sizes(gpu_available, monkeypatch, block_cells, stage):
+    # the flux block size changes which faces are duplicated on block borders
+    # and how many cells one thread updates; staging changes where a block
+    # reads its own cells' records (LDS or global memory); none changes the
+    # arithmetic
     monkeypatch.setenv("PLBFS_BLOCK_CELLS", str(block_cells))
+    monkeypatch.setenv("PLBFS_STAGE", str(stage))
     pm = channel_mesh(16, 8, 8, tets=True)
     case = pc.channel_case(re=40.0, length_scale=6.0)
     case.lsq_exact_det = 1
