@@ -60,6 +60,16 @@ __device__ __forceinline__ T wave_sum(T x) {
 
 // Least-squares gradient of cell i (src/gradients.cpp:319-389): folded stencil
 // weights times neighbour differences; ghost neighbours from the owner's patch.
+// Passes a value through an empty asm statement (no instruction is emitted).
+// Without it the optimiser merges "load from LDS" and "load from global" on
+// the two sides of a branch into one flat load through a selected pointer,
+// which is slower than either and waits on both memory counters.
+template <typename V>
+__device__ __forceinline__ V opaque4(V v) {
+    asm volatile("" : "+v"(v.x), "+v"(v.y), "+v"(v.z), "+v"(v.w));
+    return v;
+}
+
 template <typename S, typename R>
 struct GlobalStates {
     const V4<S> *__restrict__ st;
@@ -75,7 +85,7 @@ struct StagedStates {
     int c0, nc;
     __device__ __forceinline__ V4<S> operator()(int j) const {
         const unsigned rel = (unsigned)(j - c0);
-        if (rel < (unsigned)nc) return sSt[rel];
+        if (rel < (unsigned)nc) return opaque4(sSt[rel]);
         return st[j];
     }
 };
@@ -146,7 +156,7 @@ struct SideLoader {
         const unsigned rel = (unsigned)(c - c0);
         if (rel < (unsigned)nc) {
             const V4<S> *q = sRec + 4 * rel;
-            unpack_side<S>(q[0], q[1], q[2], q[3], p, g);
+            unpack_side<S>(opaque4(q[0]), opaque4(q[1]), opaque4(q[2]), opaque4(q[3]), p, g);
         } else {
             const size_t r = rec(c);
             const V4<S> *gc = grad + 3 * r;
@@ -155,7 +165,7 @@ struct SideLoader {
     }
     __device__ __forceinline__ double4 centroid(int c) const {
         const unsigned rel = (unsigned)(c - c0);
-        if (rel < (unsigned)nc) return sCg[rel];
+        if (rel < (unsigned)nc) return opaque4(sCg[rel]);
         return cgeo[c];
     }
 };
