@@ -48,7 +48,7 @@ struct CaseCtl {
 };
 
 template <typename S>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) void kb_grad(
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) void kb_grad(
     DevMesh<S> m, BatchArgs a, const Ctrl *__restrict__ ctrl, const V4<S> *__restrict__ st,
     V4<S> *__restrict__ grad) {
     if (ctrl->done) return;

@@ -31,21 +31,6 @@ struct Ctrl {
     double res[4];
 };
 
-struct IbmDev {
-    int n_nodes;
-    int kernel;  // 0 hat2, 1 peskin4
-    double h, inv_h, dt, stiffness, node_area, margin;
-    double *x, *x0, *xref, *vel, *force;  // [3][n] component-major
-    double *norm;                           // [n] normaliser of the spreading weights
-    const int32_t *np_off, *np_cell;        // node-major pairs (candidates)
-    int64_t n_pairs;
-    const int32_t *cp_off, *cp_pair;        // affected cell -> pair ids (cell-major)
-    const int32_t *cp_cell;                 // affected cell ids
-    int n_affected;
-    const double *cell_c;                   // [3][N] centroids, new order
-    const double *vol;                      // [N]
-};
-
 // Per-cell records gathered by neighbours are stored array-of-structures so a
 // gather touches one 32 B (state) or 96 B (gradient) run instead of 4 or 12
 // separate cache lines.
@@ -95,6 +80,7 @@ struct DevMesh {
     double g0x, g0y, g0z;  // centroid of the ghost of boundary face 0
     const int32_t *g_off, *g_ent;
     const int32_t *b_patch;
+    const int32_t *b_partner;  // periodic partner cell of each boundary face, -1 elsewhere
     const S *b_dmag;
     const DevPatch *patches;  // FP32 storage: rho values shifted by -rho0
     const S *corr;            // FP32 storage: [3][N] rho0/3 sum(+-n A)/V per cell
@@ -111,10 +97,12 @@ struct DevParams {
 // Ghost cells (src/Solution.cpp:759-833, src/gradients.cpp:395-504,592-648)
 // ---------------------------------------------------------------------------
 
+// pp: state of the periodic partner (read only by type 3).
 template <typename T>
-__device__ __forceinline__ void ghost_state(const DevPatch &P, const T po[4], T pg[4]) {
+__device__ __forceinline__ void ghost_state(const DevPatch &P, const T po[4], const T pp[4], T pg[4]) {
     const int rt = P.rho_type, vt = P.vel_type;
     if (rt == 1) pg[0] = (T)P.rho - (po[0] - (T)P.rho);
+    else if (rt == 3) pg[0] = pp[0];
     else if (rt == 6 || rt == 7) pg[0] = (T)P.rho;
     else pg[0] = po[0];  // 2 (neumann, dx = 0) and 8 (symmetry)
     if (vt == 1) {
@@ -129,6 +117,10 @@ __device__ __forceinline__ void ghost_state(const DevPatch &P, const T po[4], T 
         pg[1] = po[1];
         pg[2] = -po[2];
         pg[3] = po[3];
+    } else if (vt == 3) {
+        pg[1] = pp[1];
+        pg[2] = pp[2];
+        pg[3] = pp[3];
     } else {
         pg[1] = po[1];
         pg[2] = po[2];
@@ -136,11 +128,12 @@ __device__ __forceinline__ void ghost_state(const DevPatch &P, const T po[4], T 
     }
 }
 
-// Ghost gradient from the owner's state and gradient.  rho_type 8 leaves the
+// Ghost gradient from the owner's state and gradient (gp: the periodic
+// partner's gradient, read only by type 3).  rho_type 8 leaves the
 // reference's zero-initialised rho slot untouched, so it is zero here too.
 template <typename T>
 __device__ __forceinline__ void ghost_grad(const DevPatch &P, const T n[3], T dmag, const T po[4], const T go[12],
-                                           T gg[12]) {
+                                           const T gp[12], T gg[12]) {
     const int rt = P.rho_type, vt = P.vel_type;
     if (rt == 1) {
         const T q = (T)2 * ((T)P.rho - po[0]) / dmag;
@@ -155,6 +148,10 @@ __device__ __forceinline__ void ghost_grad(const DevPatch &P, const T n[3], T dm
         gg[0] = go[0];
         gg[1] = go[1];
         gg[2] = go[2];
+    } else if (rt == 3) {
+        gg[0] = gp[0];
+        gg[1] = gp[1];
+        gg[2] = gp[2];
     } else {
         gg[0] = gg[1] = gg[2] = (T)0;
     }
@@ -175,6 +172,9 @@ __device__ __forceinline__ void ghost_grad(const DevPatch &P, const T n[3], T dm
             gg[6 + c] = go[6 + c] * ((T)1 - n[c]);
             gg[9 + c] = go[9 + c] * ((T)1 - n[c]);
         }
+    } else if (vt == 3) {
+#pragma unroll
+        for (int q = 3; q < 12; q++) gg[q] = gp[q];
     } else {
 #pragma unroll
         for (int q = 3; q < 12; q++) gg[q] = (T)0;

@@ -420,6 +420,8 @@ int create_ctx(const plbfs_mesh_view *mesh, const plbfs_params *params, const pl
     }
     std::vector<int32_t> bpatch = L.b_patch;
     if (bpatch.empty()) bpatch.push_back(0);
+    std::vector<int32_t> bpartner = L.b_partner;
+    if (bpartner.empty()) bpartner.push_back(-1);
     std::vector<double> bdmag = L.b_dmag;
     if (bdmag.empty()) bdmag.push_back(1.0);
     d.corr = nullptr;
@@ -430,7 +432,8 @@ int create_ctx(const plbfs_mesh_view *mesh, const plbfs_params *params, const pl
         (rc = upload(c, (int32_t **)&d.it_own, L.it_own)) || (rc = upload(c, (int32_t **)&d.it_nbr, L.it_nbr)) ||
         (rc = upload(c, (uint32_t **)&d.it_side, L.it_side)) ||
         (rc = upload(c, (int32_t **)&d.g_off, L.g_off)) || (rc = upload(c, (int32_t **)&d.g_ent, L.g_ent)) ||
-        (rc = upload(c, (int32_t **)&d.b_patch, bpatch)) || (rc = upload(c, (DevPatch **)&d.patches, dp)))
+        (rc = upload(c, (int32_t **)&d.b_patch, bpatch)) || (rc = upload(c, (int32_t **)&d.b_partner, bpartner)) ||
+        (rc = upload(c, (DevPatch **)&d.patches, dp)))
         return fail(rc);
     if (!c->f32) {
         // n, A, dt only; offsets come from the face and cell centroids
@@ -458,7 +461,7 @@ int create_ctx(const plbfs_mesh_view *mesh, const plbfs_params *params, const pl
         e.st_cnt = d.st_cnt; e.st_nbr = d.st_nbr; e.vol = d.vol; e.inv_vol = d.inv_vol; e.cfl_area = d.cfl_area;
         e.blk_item_off = d.blk_item_off; e.it_own = d.it_own; e.it_nbr = d.it_nbr; e.g_off = d.g_off;
         e.it_side = d.it_side;
-        e.g_ent = d.g_ent; e.b_patch = d.b_patch;
+        e.g_ent = d.g_ent; e.b_patch = d.b_patch; e.b_partner = d.b_partner;
         auto to32 = [](const std::vector<double> &v) { return std::vector<float>(v.begin(), v.end()); };
         std::vector<DevPatch> dp32 = dp;
         for (auto &q : dp32) q.rho -= p.ref_rho;

@@ -110,8 +110,9 @@ __device__ __forceinline__ void lsq_grad(const DevMesh<S> &m, int i, const ST &s
                 pj[2] = qj.z;
                 pj[3] = qj.w;
             } else {
+                // (periodic ghosts were replaced by their partner cell in st_nbr)
                 const DevPatch P = m.patches[m.b_patch[-1 - j]];
-                ghost_state<S>(P, p0, pj);
+                ghost_state<S>(P, p0, p0, pj);
             }
             const S wx = m.st_w[((size_t)s * 3 + 0) * N + i];
             const S wy = m.st_w[((size_t)s * 3 + 1) * N + i];
@@ -229,8 +230,10 @@ __device__ __forceinline__ void item_flux(const DevMesh<S> &m, const DevParams &
             ld.side(nb, pn, gN);
         } else {
             const DevPatch P = m.patches[m.b_patch[0]];
-            ghost_state<S>(P, po, pn);
-            ghost_grad<S>(P, n, m.b_dmag[0], po, go, gN);
+            S pp[4], gp[12];
+            if (P.rho_type == 3 || P.vel_type == 3) ld.side(m.b_partner[0], pp, gp);
+            ghost_state<S>(P, po, pp, pn);
+            ghost_grad<S>(P, n, m.b_dmag[0], po, go, gp, gN);
         }
 #pragma unroll
         for (int q = 0; q < 4; q++) {
@@ -241,7 +244,12 @@ __device__ __forceinline__ void item_flux(const DevMesh<S> &m, const DevParams &
     } else {
         const int b = -1 - nb;
         const DevPatch P = m.patches[m.b_patch[b]];
-        ghost_grad<S>(P, n, m.b_dmag[b], po, go, gN);
+        S gp[12];
+        if (P.rho_type == 3 || P.vel_type == 3) {
+            S pp[4];
+            ld.side(m.b_partner[b], pp, gp);
+        }
+        ghost_grad<S>(P, n, m.b_dmag[b], po, go, gp, gN);
         S fv[4];
         if (P.rho_type == 1) fv[0] = (S)P.rho;
         else if (P.rho_type == 8) fv[0] = po[0];

@@ -239,6 +239,10 @@ extern "C" int plbfs_case_patches(const plbfs_case *c, const plbfs_mesh *m, plbf
         out[q].v = b->v;
         out[q].w = b->w;
         out[q].is_empty = std::strcmp(buf, "empty") == 0;
+        // one translation for every periodic patch (src/preprocessor.cpp:299-301)
+        out[q].per_translate[0] = c->n("uns_boundary_conditions/periodic_translation/x", 0.0);
+        out[q].per_translate[1] = c->n("uns_boundary_conditions/periodic_translation/y", 0.0);
+        out[q].per_translate[2] = c->n("uns_boundary_conditions/periodic_translation/z", 0.0);
     }
     return PLBFS_OK;
 }

@@ -47,8 +47,74 @@ uint64_t hilbert_key3(uint32_t x, uint32_t y, uint32_t z, int bits) {
     return key;
 }
 
-static bool rho_type_ok(int t) { return t == 1 || t == 2 || t == 6 || t == 7 || t == 8; }
-static bool vel_type_ok(int t) { return t == 1 || t == 2 || t == 6 || t == 8; }
+static bool rho_type_ok(int t) { return t == 1 || t == 2 || t == 3 || t == 6 || t == 7 || t == 8; }
+static bool vel_type_ok(int t) { return t == 1 || t == 2 || t == 3 || t == 6 || t == 8; }
+
+// Interior cell nearest to x (file index; ties to the smaller index), found
+// through a uniform bin grid of the interior centroids.
+struct NearestCell {
+    double lo[3], hb;
+    int nb[3];
+    std::vector<int32_t> off, idx;
+    const double *cc;
+    int N;
+    NearestCell(const double *cc_, int N_) : cc(cc_), N(N_) {
+        double hi[3];
+        for (int a = 0; a < 3; a++) lo[a] = hi[a] = cc[a];
+        for (int i = 0; i < N; i++)
+            for (int a = 0; a < 3; a++) {
+                lo[a] = std::min(lo[a], cc[3 * i + a]);
+                hi[a] = std::max(hi[a], cc[3 * i + a]);
+            }
+        double vol = 1;
+        for (int a = 0; a < 3; a++) vol *= std::max(hi[a] - lo[a], 1e-12);
+        hb = std::max(std::cbrt(vol / N) * 2.0, 1e-12);
+        for (int a = 0; a < 3; a++) nb[a] = std::max(1, (int)std::ceil((hi[a] - lo[a]) / hb) + 1);
+        const size_t nbin = (size_t)nb[0] * nb[1] * nb[2];
+        off.assign(nbin + 1, 0);
+        std::vector<size_t> bin(N);
+        for (int i = 0; i < N; i++) {
+            bin[i] = key(cc + 3 * i);
+            off[bin[i] + 1]++;
+        }
+        for (size_t k = 0; k < nbin; k++) off[k + 1] += off[k];
+        idx.resize(N);
+        std::vector<int32_t> fill(off.begin(), off.end() - 1);
+        for (int i = 0; i < N; i++) idx[fill[bin[i]]++] = i;
+    }
+    int coord(double x, int a) const { return std::min(nb[a] - 1, std::max(0, (int)std::floor((x - lo[a]) / hb))); }
+    size_t key(const double *x) const {
+        return ((size_t)coord(x[0], 0) * nb[1] + coord(x[1], 1)) * nb[2] + coord(x[2], 2);
+    }
+    int find(const double x[3]) const {
+        const int c[3] = {coord(x[0], 0), coord(x[1], 1), coord(x[2], 2)};
+        double best = 1e300;
+        int arg = -1;
+        for (int r = 0;; r++) {
+            // shell of bins at Chebyshev distance r
+            for (int i = c[0] - r; i <= c[0] + r; i++)
+                for (int j = c[1] - r; j <= c[1] + r; j++)
+                    for (int k = c[2] - r; k <= c[2] + r; k++) {
+                        if (std::max(std::abs(i - c[0]), std::max(std::abs(j - c[1]), std::abs(k - c[2]))) != r)
+                            continue;
+                        if (i < 0 || j < 0 || k < 0 || i >= nb[0] || j >= nb[1] || k >= nb[2]) continue;
+                        const size_t b = ((size_t)i * nb[1] + j) * nb[2] + k;
+                        for (int q = off[b]; q < off[b + 1]; q++) {
+                            const int n = idx[q];
+                            const double dx = cc[3 * n] - x[0], dy = cc[3 * n + 1] - x[1], dz = cc[3 * n + 2] - x[2];
+                            const double d = std::sqrt(std::pow(dx, 2) + std::pow(dy, 2) + std::pow(dz, 2));
+                            if (d < best || (d == best && n < arg)) {
+                                best = d;
+                                arg = n;
+                            }
+                        }
+                    }
+            // every unvisited bin is at least r * hb away from x's bin
+            if (arg >= 0 && best < r * hb) return arg;
+            if (r > nb[0] + nb[1] + nb[2]) return arg;
+        }
+    }
+};
 
 int build_layout(const plbfs_mesh_view &m, const plbfs_params &p, const plbfs_patch *patches, int n_patches,
                  const int32_t *bface_patch, int block_cells, Layout &L) {
@@ -61,6 +127,8 @@ int build_layout(const plbfs_mesh_view &m, const plbfs_params &p, const plbfs_pa
         if (!rho_type_ok(patches[q].rho_type) || !vel_type_ok(patches[q].vel_type))
             return plbfs_fail(PLBFS_ERR_BCTYPE, "patch %d: unsupported bc types rho=%d vel=%d", q,
                               patches[q].rho_type, patches[q].vel_type);
+        if ((patches[q].rho_type == 3) != (patches[q].vel_type == 3))
+            return plbfs_fail(PLBFS_ERR_BCTYPE, "patch %d: periodic must apply to both rho and velocity", q);
     }
     for (int b = 0; b < G; b++)
         if (bface_patch[b] < 0 || bface_patch[b] >= n_patches)
@@ -264,6 +332,32 @@ int build_layout(const plbfs_mesh_view &m, const plbfs_params &p, const plbfs_pa
         L.corr.resize((size_t)3 * N);
         for (int n = 0; n < N; n++)
             for (int a = 0; a < 3; a++) L.corr[(size_t)a * N + n] = cf[3 * L.perm[n] + a];
+    }
+
+    // periodic partners: the interior cell nearest to the owner's centroid
+    // + per_translate (src/Boundary_Conditions.cpp:425-445, owner in place of
+    // the reference's uninitialised index); -1 on other patches
+    L.b_partner.assign(G, -1);
+    {
+        bool any = false;
+        for (int q = 0; q < n_patches; q++) any = any || patches[q].rho_type == 3 || patches[q].vel_type == 3;
+        if (any) {
+            NearestCell nc(cc, N);
+            for (int b = 0; b < G; b++) {
+                const plbfs_patch &P = patches[bface_patch[b]];
+                if (P.rho_type != 3 && P.vel_type != 3) continue;
+                const int o = m.owner[NI + b];
+                const double x[3] = {cc[3 * o] + P.per_translate[0], cc[3 * o + 1] + P.per_translate[1],
+                                     cc[3 * o + 2] + P.per_translate[2]};
+                L.b_partner[b] = L.inv[nc.find(x)];
+            }
+            // a periodic ghost is its partner (state and gradient), so the
+            // gradient stencil reads the partner directly
+            for (size_t k = 0; k < L.st_nbr.size(); k++) {
+                const int j = L.st_nbr[k];
+                if (j < 0 && L.b_partner[-1 - j] >= 0) L.st_nbr[k] = L.b_partner[-1 - j];
+            }
+        }
     }
 
     // boundary faces

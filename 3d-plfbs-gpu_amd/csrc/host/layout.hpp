@@ -60,6 +60,7 @@ struct Layout {
     std::vector<int32_t> b_patch;  // [G]
     std::vector<double> b_dmag;    // [G] |x_ghost - x_owner|
     std::vector<int32_t> b_owner;  // [G] new owner index
+    std::vector<int32_t> b_partner;  // [G] periodic partner (new index), -1 elsewhere
     // LSQ inverse and RHS in the reference layout, file order (for tests)
     std::vector<double> lsq_inv9;  // [N*9]
     std::vector<double> lsq_rhs;   // [N*6*3]

@@ -51,7 +51,7 @@ class MeshView(C.Structure):
 
 class Patch(C.Structure):
     _fields_ = [("rho_type", C.c_int32), ("vel_type", C.c_int32), ("rho", C.c_double), ("u", C.c_double),
-                ("v", C.c_double), ("w", C.c_double), ("is_empty", C.c_int32)]
+                ("v", C.c_double), ("w", C.c_double), ("is_empty", C.c_int32), ("per_translate", C.c_double * 3)]
 
 
 class Params(C.Structure):

@@ -46,6 +46,9 @@ class Case:
     precision: int = 0
     check_interval: int = 10
     lsq_exact_det: int = 0
+    # uns_boundary_conditions/periodic_translation: one vector for every periodic
+    # patch (src/preprocessor.cpp:299-301); a bc dict may override it per patch
+    periodic_translation: tuple = (0.0, 0.0, 0.0)
 
     @property
     def factor(self):
@@ -81,7 +84,8 @@ class Case:
             out.append(dict(name=name, rho_type=BC_CODES[b["rho_type"]], vel_type=BC_CODES[b["vel_type"]],
                             rho=float(b.get("rho", 1.0)), u=float(b.get("u", 0.0)) * f,
                             v=float(b.get("v", 0.0)) * f, w=float(b.get("w", 0.0)) * f,
-                            is_empty=int(name == "empty")))
+                            is_empty=int(name == "empty"),
+                            per_translate=tuple(float(t) for t in b.get("per_translate", self.periodic_translation))))
         return out
 
     def initial_state(self, cx, cy, cz):

@@ -107,7 +107,8 @@ def make_params(p):
 def make_patches(patches):
     arr = (nat.Patch * len(patches))()
     for q, P in enumerate(patches):
-        arr[q] = nat.Patch(P["rho_type"], P["vel_type"], P["rho"], P["u"], P["v"], P["w"], int(P.get("is_empty", 0)))
+        arr[q] = nat.Patch(P["rho_type"], P["vel_type"], P["rho"], P["u"], P["v"], P["w"], int(P.get("is_empty", 0)),
+                           (C.c_double * 3)(*P.get("per_translate", (0.0, 0.0, 0.0))))
     return arr
 
 

@@ -76,13 +76,18 @@ typedef struct {
 } plbfs_mesh_view;
 
 /* One boundary patch (src/Boundary_Conditions.cpp:388-452): types use the
- * reference codes 1 dirichlet, 2 neumann, 6 inlet, 7 wall (rho only),
- * 8 symmetry-x.  Velocities are already multiplied by mach_no/scale. */
+ * reference codes 1 dirichlet, 2 neumann, 3 periodic, 6 inlet, 7 wall (rho
+ * only), 8 symmetry-x.  Velocities are already multiplied by mach_no/scale.
+ * A periodic face's ghost takes the state and gradient of its partner: the
+ * interior cell nearest to the owner's centroid + per_translate
+ * (src/Boundary_Conditions.cpp:425-445, whose search starts from an
+ * uninitialised cell index; the owner is used here). */
 typedef struct {
     int32_t rho_type;
     int32_t vel_type;
     double rho, u, v, w;
     int32_t is_empty;           /* OpenFOAM patch type "empty" (src/Solver.cpp:308) */
+    double per_translate[3];    /* periodic_translation (src/preprocessor.cpp:299-301) */
 } plbfs_patch;
 
 /* Run parameters (the parts of global_variables / initial_conditions the

@@ -81,6 +81,7 @@ typedef struct {
     int rho_type, vel_type;        /* reference codes 1..9 (preprocessor.cpp:243-290) */
     double rho, u, v, w;           /* BC values, velocity already scaled by mach factor */
     int is_empty;                  /* OpenFOAM patch type "empty" */
+    double per_translate[3];       /* periodic (type 3): partner = nearest cell to owner + this */
 } om_patch;
 
 typedef struct {
@@ -108,6 +109,7 @@ typedef struct {
     om_patch *patch;
     int N, G, T, F, NI;            /* cells, ghosts, total, faces, internal faces */
     int *nbr;                      /* [F] neighbour incl. ghost index for boundary faces */
+    int *partner;                  /* [G] periodic partner cell (Boundary_Conditions.cpp:428-445), else 0 */
     /* geometry */
     double *cx, *cy, *cz, *vol, *tot_area;   /* [T] */
     double *fx, *fy, *fz, *fa, *ni, *nj, *nk, *fdt; /* [F] */
@@ -338,14 +340,42 @@ static void build_cfl_areas(om_ctx *c) {
 }
 
 /* ------------------------------------------------------------------------- */
+/* Periodic partners: Boundary_Conditions.cpp:425-445.  The reference takes   */
+/* the centroid of an uninitialised index there; the owner of the boundary    */
+/* face is used here.  Partner = the interior cell nearest to owner centroid  */
+/* + per_translate (first minimum in cell order); 0 for other patches, the    */
+/* reference's default.                                                       */
+/* ------------------------------------------------------------------------- */
+
+static void build_partners(om_ctx *c) {
+    c->partner = xcalloc(c->G ? c->G : 1, sizeof(int));
+    for (int b = 0; b < c->G; b++) {
+        const om_patch *P = &c->patch[c->m.bface_patch[b]];
+        if (P->rho_type != 3 && P->vel_type != 3) continue;
+        int o = c->m.owner[c->NI + b];
+        double x = c->cx[o] + P->per_translate[0], y = c->cy[o] + P->per_translate[1],
+               z = c->cz[o] + P->per_translate[2];
+        double best = 10000000;
+        for (int i = 0; i < c->N; i++) {
+            double dx = c->cx[i] - x, dy = c->cy[i] - y, dz = c->cz[i] - z;
+            double d = sqrt(pow(dx, 2) + pow(dy, 2) + pow(dz, 2));
+            if (d < best) {
+                best = d;
+                c->partner[b] = i;
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------- */
 /* Creation                                                                  */
 /* ------------------------------------------------------------------------- */
 
 static int check_patch_types(const om_patch *p, int n) {
     for (int i = 0; i < n; i++) {
         int r = p[i].rho_type, v = p[i].vel_type;
-        if (!(r == 1 || r == 2 || r == 6 || r == 7 || r == 8)) return OM_ERR_BCTYPE;
-        if (!(v == 1 || v == 2 || v == 6 || v == 8)) return OM_ERR_BCTYPE;
+        if (!(r == 1 || r == 2 || r == 3 || r == 6 || r == 7 || r == 8)) return OM_ERR_BCTYPE;
+        if (!(v == 1 || v == 2 || v == 3 || v == 6 || v == 8)) return OM_ERR_BCTYPE;
     }
     return OM_OK;
 }
@@ -394,12 +424,13 @@ om_ctx *om_create(const om_mesh_in *m, const om_params *p, const om_patch *patch
     build_geometry(c);
     build_lsq(c);
     build_cfl_areas(c);
+    build_partners(c);
     return c;
 }
 
 void om_destroy(om_ctx *c) {
     if (!c) return;
-    void *ptrs[] = {c->patch, c->nbr, c->cx, c->cy, c->cz, c->vol, c->tot_area, c->fx, c->fy, c->fz, c->fa,
+    void *ptrs[] = {c->patch, c->nbr, c->partner, c->cx, c->cy, c->cz, c->vol, c->tot_area, c->fx, c->fy, c->fz, c->fa,
                     c->ni, c->nj, c->nk, c->fdt, c->st_off, c->st_cell, c->st_face, c->Lxx, c->Lxy, c->Lxz,
                     c->Lyx, c->Lyy, c->Lyz, c->Lzx, c->Lzy, c->Lzz, c->Rx, c->Ry, c->Rz, c->cfl_area,
                     c->s_rho, c->s_u, c->s_v, c->s_w, c->t_rho, c->t_u, c->t_v, c->t_w, c->a_rho, c->a_u,
@@ -452,6 +483,8 @@ void om_get_lsq(const om_ctx *c, double *inv9, double *rhs, double *cfl_area) {
     }
 }
 
+void om_get_partner(const om_ctx *c, int *out) { memcpy(out, c->partner, sizeof(int) * (size_t)c->G); }
+
 void om_get_stencil(const om_ctx *c, int *off, int *cell, int *face) {
     memcpy(off, c->st_off, sizeof(int) * (c->N + 1));
     memcpy(cell, c->st_cell, sizeof(int) * c->st_off[c->N]);
@@ -495,11 +528,12 @@ void om_set_body_force(om_ctx *c, const double *fxv, const double *fyv, const do
 
 static void update_bcs(om_ctx *c, double *rho, double *u, double *v, double *w) {
     for (int b = 0; b < c->G; b++) {
-        int j = b + c->N, face = b + c->NI, nb = c->m.owner[face];
+        int j = b + c->N, face = b + c->NI, nb = c->m.owner[face], pn = c->partner[b];
         const om_patch *P = &c->patch[c->m.bface_patch[b]];
         double dx = 0;
         if (P->rho_type == 1) rho[j] = P->rho - (rho[nb] - P->rho);
         else if (P->rho_type == 2) rho[j] = rho[nb] + dx * P->rho;
+        else if (P->rho_type == 3) rho[j] = rho[pn];
         else if (P->rho_type == 6) rho[j] = P->rho;
         else if (P->rho_type == 7) rho[j] = P->rho;
         else if (P->rho_type == 8) rho[j] = rho[nb];
@@ -511,6 +545,10 @@ static void update_bcs(om_ctx *c, double *rho, double *u, double *v, double *w) 
             u[j] = u[nb] + dx * P->u;
             v[j] = v[nb] + dx * P->v;
             w[j] = w[nb] + dx * P->w;
+        } else if (P->vel_type == 3) {
+            u[j] = u[pn];
+            v[j] = v[pn];
+            w[j] = w[pn];
         } else if (P->vel_type == 6) {
             double taper = 1.0;
             u[j] = P->u * taper;
@@ -586,6 +624,10 @@ static void compute_gradients(om_ctx *c, const double *rho, const double *u, con
             GI(j, 0) = GI(o, 0); GI(j, 1) = GI(o, 1); GI(j, 2) = GI(o, 2);
         } else if (P->rho_type == 8) {
             ghost_grads_dirichlet(c, rho, u, v, w, b, o, 0);
+        } else {
+            /* periodic: the partner's gradient (gradients.cpp:434-440) */
+            const int pn = c->partner[b];
+            GI(j, 0) = GI(pn, 0); GI(j, 1) = GI(pn, 1); GI(j, 2) = GI(pn, 2);
         }
         if (P->vel_type == 1) {
             ghost_grads_dirichlet(c, rho, u, v, w, b, o, 0);
@@ -599,6 +641,10 @@ static void compute_gradients(om_ctx *c, const double *rho, const double *u, con
             for (int q = 3; q < 12; q++) GI(j, q) = 0.0;
         } else if (P->vel_type == 8) {
             ghost_grads_dirichlet(c, rho, u, v, w, b, o, 0);
+        } else {
+            /* periodic (gradients.cpp:488-502) */
+            const int pn = c->partner[b];
+            for (int q = 3; q < 12; q++) GI(j, q) = GI(pn, q);
         }
     }
 }

@@ -29,7 +29,7 @@ class MeshIn(C.Structure):
 
 class Patch(C.Structure):
     _fields_ = [("rho_type", C.c_int), ("vel_type", C.c_int), ("rho", C.c_double), ("u", C.c_double),
-                ("v", C.c_double), ("w", C.c_double), ("is_empty", C.c_int)]
+                ("v", C.c_double), ("w", C.c_double), ("is_empty", C.c_int), ("per_translate", C.c_double * 3)]
 
 
 class Params(C.Structure):
@@ -61,6 +61,7 @@ def lib():
         L.om_get_faces.argtypes = [vp, dp, dp, dp, dp, dp, dp, dp, dp, ip]
         L.om_get_lsq.argtypes = [vp, dp, dp, dp]
         L.om_get_stencil.argtypes = [vp, ip, ip, ip]
+        L.om_get_partner.argtypes = [vp, ip]
         L.om_set_state.argtypes = [vp, dp, dp, dp, dp]
         L.om_get_state.argtypes = [vp, dp, dp, dp, dp]
         L.om_get_residual.argtypes = [vp, dp]
@@ -107,7 +108,7 @@ class Oracle:
         patches = (Patch * len(case.patches))()
         for q, P in enumerate(case.patches):
             patches[q] = Patch(P["rho_type"], P["vel_type"], P["rho"], P["u"], P["v"], P["w"],
-                               int(P.get("is_empty", 0)))
+                               int(P.get("is_empty", 0)), (C.c_double * 3)(*P.get("per_translate", (0, 0, 0))))
         self._patches = patches
         err = C.c_int(0)
         self.h = L.om_create(C.byref(self.mi), C.byref(self.params), patches, len(case.patches), C.byref(err))
@@ -153,6 +154,11 @@ class Oracle:
         face = np.zeros(n, dtype=np.int32)
         lib().om_get_stencil(self.h, _p(off, C.c_int), _p(cell, C.c_int), _p(face, C.c_int))
         return off, cell, face
+
+    def partner(self):
+        out = np.zeros(max(self.T - self.N, 1), dtype=np.int32)
+        lib().om_get_partner(self.h, _p(out, C.c_int))
+        return out[: self.T - self.N]
 
     def set_state(self, rho, u, v, w):
         arrs = [np.ascontiguousarray(x, dtype=np.float64) for x in (rho, u, v, w)]

@@ -45,3 +45,22 @@ def rel_err(a, b):
     a = np.asarray(a, dtype=np.float64)
     b = np.asarray(b, dtype=np.float64)
     return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+PERIODIC_Z_PATCHES = {"x0": "inlet", "x1": "outlet", "y0": "farfield", "y1": "farfield", "z0": "zlow", "z1": "zhigh"}
+
+
+def periodic_z_case(lz, re=40.0, length_scale=4.0):
+    """Channel whose z sides are periodic: the z0 ghosts take the cells
+    next to z1 (translate +lz) and vice versa."""
+    case = pc.channel_case(re=re, length_scale=length_scale)
+    per = dict(rho=1.0, u=0.0, v=0.0, w=0.0, vel_type="periodic", rho_type="periodic")
+    case.bcs["zlow"] = dict(per, per_translate=(0.0, 0.0, lz))
+    case.bcs["zhigh"] = dict(per, per_translate=(0.0, 0.0, -lz))
+    return case
+
+
+def periodic_z_mesh(nx=10, ny=5, nz=6, jitter=0.15, seed=21, tets=False, permute=True):
+    return meshgen.box_mesh(nx, ny, nz, lengths=(float(nx), float(ny), float(nz)), tets=tets, jitter=jitter,
+                            jitter_seed=seed, permute_seed=seed + 1 if permute else None,
+                            patch_types=PERIODIC_Z_PATCHES)

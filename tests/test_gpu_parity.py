@@ -92,6 +92,23 @@ def test_channel_100_steps(gpu_available, tets):
     assert max(errs) < 1e-10, errs
 
 
+@pytest.mark.parametrize("tets", [False, True])
+def test_periodic_z_steps(gpu_available, tets):
+    from helpers import periodic_z_case, periodic_z_mesh
+    pm = periodic_z_mesh(10, 5, 6, tets=tets)
+    case = periodic_z_case(6.0)
+    case.lsq_exact_det = 1 if tets else 0
+    o, hm, s = make_pair(pm, case)
+    st = perturbed_state(case, o, amp=0.01, seed=13)
+    o.set_state(*st)
+    s.set_state(*st)
+    n = 60 if tets else 100
+    assert o.run(n)["rc"] == 0
+    assert s.run(n)["rc"] == 0
+    errs = field_errs(s.state(), o.state())
+    assert max(errs) < 1e-10, errs
+
+
 def test_flux_block_too_large_for_lds_is_a_clean_error(gpu_available, monkeypatch):
     # 1024 FP64 tets need ~2300 face slots x 32 B > 64 KiB of LDS
     monkeypatch.setenv("PLBFS_BLOCK_CELLS", "1024")
