@@ -500,6 +500,31 @@ extern "C" int plbfs_program_run(const char *xml_path, plbfs_result *result) {
     const std::string plt = outdir + "/plt/" + std::to_string(r.total_steps) + ".dat";
     if ((rc = plbfs_write_tecplot(m, plt.c_str(), sr.data(), su.data(), sv.data(), sw.data(), r.total_steps)))
         return rc;
+    // forces: every "wall" patch with the reference's coefficients
+    // (src/post_processing.cpp:170-171), and the immersed objects
+    if (FILE *f = std::fopen((outdir + "/drag.txt").c_str(), "w")) {
+        const double rho0 = p.ref_rho, U = p.max_velocity;
+        const double X = c->n("geometry/x", 1.0), Y = c->n("geometry/y", 1.0), Z = c->n("geometry/z", 1.0);
+        std::fprintf(f, "patch , drag , lift , drag_coef , lift_coef\n");
+        char buf[256];
+        for (int q = 0; q < NP; q++) {
+            plbfs_mesh_patch_type(m, q, buf, sizeof buf);
+            if (std::strcmp(buf, "wall") != 0) continue;
+            double dl[2];
+            if ((rc = plbfs_wall_forces(ctx, &view, plbfs_mesh_bface_patch(m), q, dl))) {
+                std::fclose(f);
+                return rc;
+            }
+            std::fprintf(f, "%d , %.17g , %.17g , %.17g , %.17g\n", q, dl[0], dl[1],
+                         2 * dl[0] / rho0 / std::pow(U, 2) / Y / Z, 2 * dl[1] / rho0 / std::pow(U, 2) / X / Z);
+        }
+        if (!c->objects.empty()) {
+            double fb[3];
+            if ((rc = plbfs_ibm_body_force(ctx, fb)) == PLBFS_OK)
+                std::fprintf(f, "immersed , %.17g , %.17g , %.17g\n", fb[0], fb[1], fb[2]);
+        }
+        std::fclose(f);
+    }
     const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
     if (FILE *f = std::fopen((outdir + "/globals.txt").c_str(), "w")) {
         // program.cpp:192-234 fields that the solver determines

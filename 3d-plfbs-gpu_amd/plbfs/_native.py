@@ -25,7 +25,7 @@ EXPORTS = [
     "plbfs_last_error", "plbfs_device_count", "plbfs_create", "plbfs_destroy", "plbfs_set_state",
     "plbfs_get_state", "plbfs_set_ibm", "plbfs_get_ibm", "plbfs_get_body_force", "plbfs_run",
     "plbfs_get_residual_history", "plbfs_eval_residual", "plbfs_get_dt", "plbfs_enable_timing",
-    "plbfs_kernel_stats", "plbfs_layout_info",
+    "plbfs_kernel_stats", "plbfs_layout_info", "plbfs_wall_forces", "plbfs_ibm_body_force",
     "plbfs_mesh_read_openfoam", "plbfs_mesh_from_arrays", "plbfs_mesh_free", "plbfs_mesh_info",
     "plbfs_mesh_view_get", "plbfs_mesh_bface_patch", "plbfs_mesh_patch_type", "plbfs_mesh_raw",
     "plbfs_mesh_n_face_verts", "plbfs_case_load", "plbfs_case_free", "plbfs_case_params",
@@ -107,6 +107,8 @@ def lib():
     L.plbfs_enable_timing.argtypes = [vp, C.c_int32]
     L.plbfs_kernel_stats.argtypes = [vp, C.c_int32, dp, ip]
     L.plbfs_layout_info.argtypes = [vp, i64p]
+    L.plbfs_wall_forces.argtypes = [vp, C.POINTER(MeshView), ip, C.c_int32, dp]
+    L.plbfs_ibm_body_force.argtypes = [vp, dp]
     L.plbfs_mesh_read_openfoam.argtypes = [C.c_char_p, C.c_double, C.c_double, C.c_double, C.POINTER(vp)]
     L.plbfs_mesh_from_arrays.argtypes = [C.c_int32, C.POINTER(C.c_float), C.POINTER(C.c_float),
                                          C.POINTER(C.c_float), C.c_int32, C.c_int32, C.c_int32, ip, ip, ip, ip,

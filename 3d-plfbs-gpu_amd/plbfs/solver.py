@@ -187,6 +187,18 @@ class GpuSolver:
         nat.check(nat.lib().plbfs_get_body_force(self.h, nat.dptr(f)))
         return f.reshape(self.N, 3)
 
+    def wall_forces(self, patch):
+        """(drag, lift) on the boundary faces of patch index `patch` (reference sums)."""
+        out = np.zeros(2)
+        nat.check(nat.lib().plbfs_wall_forces(self.h, C.byref(self.mesh.view), nat.iptr(self._bp) if self._bp.size
+                                              else None, int(patch), nat.dptr(out)))
+        return out
+
+    def ibm_body_force(self):
+        out = np.zeros(3)
+        nat.check(nat.lib().plbfs_ibm_body_force(self.h, nat.dptr(out)))
+        return out
+
     def enable_timing(self, on=True):
         nat.check(nat.lib().plbfs_enable_timing(self.h, 1 if on else 0))
 

@@ -186,6 +186,19 @@ int plbfs_kernel_stats(plbfs_ctx *ctx, int32_t which, double *total_ms, int32_t 
  * block, gather entries, cells per flux block, LDS bytes per flux block. */
 int plbfs_layout_info(plbfs_ctx *ctx, int64_t *info10);
 
+/* ---- Post-processing (src/post_processing.cpp:50-173) ---------------------
+ * Surface force on the boundary faces of one patch from the current state,
+ * with the reference's sums over "wall" faces: p = rho_owner / 3 gamma,
+ * nu' = nu gamma, ghost-cell gradients of u and v,
+ *   drag = sum ((p - nu' du/dx) n_x - nu' du/dy n_y) A,
+ *   lift = sum (nu' dv/dx n_x - (p - nu' dv/dy) n_y) A.
+ * mesh and bface_patch are the arrays given to plbfs_create. */
+int plbfs_wall_forces(plbfs_ctx *ctx, const plbfs_mesh_view *mesh, const int32_t *bface_patch, int32_t patch,
+                      double *drag_lift2);
+/* Force of the fluid on the immersed nodes: minus the sum of their spring
+ * forces (the sphere's drag and lift for a body held by the springs). */
+int plbfs_ibm_body_force(plbfs_ctx *ctx, double *force3);
+
 /* ---- Batched cases ----------------------------------------------------------
  * B independent cases on one mesh topology, advanced together: the mesh,
  * layout and LSQ weights are shared; each case has its own viscosity (its

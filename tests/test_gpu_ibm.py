@@ -70,3 +70,23 @@ def test_ibm_rejects_far_nodes(gpu_available):
     with pytest.raises(nat.PlbfsError) as e:
         s.set_ibm(X, X + 5.0, stiffness=1.0, node_area=1.0, dt=0.5, h=1.0, kernel=1)
     assert e.value.rc == -7
+
+
+def test_sphere_drag_points_downstream(gpu_available):
+    """A sphere held by springs in a uniform stream: the fluid pushes it in +x
+    (minus the sum of the spring forces), and lift is small against drag."""
+    from helpers import channel_mesh
+    from plbfs import objects
+    pm = channel_mesh(16, 8, 8, jitter=0.1)
+    case = pc.channel_case(re=20.0, length_scale=3.0)
+    hm = HostMesh.from_polymesh(pm, case.visc, case.gamma, case.fneq_min)
+    s = GpuSolver(hm, case.params, case.patches_for(pm.patch_names))
+    cells = hm.geometry()
+    s.set_state(*case.initial_state(cells["cx"], cells["cy"], cells["cz"]))
+    X, dA = objects.fibonacci_sphere(100, (5.0, 4.0, 4.0), 1.5)
+    s.set_ibm(X, X, stiffness=5.0, node_area=dA, dt=case.time_marching_step, h=1.0, kernel=1)
+    assert s.run(300)["rc"] == 0
+    F = s.ibm_body_force()
+    _, _, fn = s.ibm_state()
+    assert np.allclose(F, -fn.sum(0))
+    assert F[0] > 0 and abs(F[1]) < 0.2 * F[0] and abs(F[2]) < 0.2 * F[0], F

@@ -232,3 +232,38 @@ def test_nan_state_reports_error(gpu_available):
     r = s.run(5)
     assert r["rc"] == -2, nat.ERR_NAMES.get(r["rc"])
     assert r["steps"] == 1
+
+
+def test_wall_forces_match_reference_sums(gpu_available):
+    """Drag and lift over each wall patch after 50 steps, against the same sums
+    (src/post_processing.cpp:96-107) evaluated from the oracle's state and
+    ghost-cell gradients."""
+    pm = cavity_mesh(6, tets=False)
+    case = pc.cavity_case()
+    o, hm, s = make_pair(pm, case)
+    st = initial_state(case, o)
+    o.set_state(*st)
+    s.set_state(*st)
+    assert o.run(50)["rc"] == 0 and s.run(50)["rc"] == 0
+    so = o.state()
+    R, go = o.eval_residual(*[np.concatenate([x, np.zeros(o.T - o.N)]) for x in so])
+    f = o.faces()
+    N, NI = pm.n_cells, pm.n_internal
+    bp = hm.bface_patch()
+    gam, nuv = case.gamma, case.visc * case.gamma
+    checked = 0
+    for q, name in enumerate(pm.patch_names):
+        drag = lift = 0.0
+        for b in np.flatnonzero(bp == q):
+            fi, ow = NI + b, pm.owner[NI + b]
+            n = (f["ni"][fi], f["nj"][fi], f["nk"][fi])
+            A = f["area"][fi]
+            p = so[0][ow] / 3.0 * gam
+            gu, gv = go[N + b, 3:6], go[N + b, 6:9]
+            drag += ((p - nuv * gu[0]) * n[0] - nuv * gu[1] * n[1]) * A
+            lift += (nuv * gv[0] * n[0] - (p - nuv * gv[1]) * n[1]) * A
+        dl = s.wall_forces(q)
+        assert abs(dl[0] - drag) <= 1e-10 * max(1.0, abs(drag)), (name, dl, drag)
+        assert abs(dl[1] - lift) <= 1e-10 * max(1.0, abs(lift)), (name, dl, lift)
+        checked += 1
+    assert checked == len(pm.patch_names)
