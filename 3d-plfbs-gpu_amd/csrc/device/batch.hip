@@ -36,6 +36,7 @@ namespace {
 
 struct BatchArgs {
     int B, C;
+    int stage;              // 1: stage the tile's own (cell, case) records in LDS
     const double *visc;     // [B] nu
     const double *v3g;      // [B] 3 nu / gamma
     const int32_t *done;    // [B] frozen cases
@@ -78,12 +79,27 @@ __global__ __launch_bounds__(kBlock) void kb_flux(DevMesh<S> m, DevParams prm, B
     const int e0 = m.blk_item_off[blk], e1 = m.blk_item_off[blk + 1];
     const int stride = m.max_items * C;
     S *sF = reinterpret_cast<S *>(lds);
+    const int c0 = blk * m.block_cells;
+    const int ncell = min(N, c0 + m.block_cells) - c0;
+    V4<S> *sRec = nullptr;
+    double4 *sCg = nullptr;
+    int nst = 0;
+    if (a.stage) {
+        char *base = reinterpret_cast<char *>(lds) + staged_offset<S>(m.max_items, C);
+        sRec = reinterpret_cast<V4<S> *>(base);
+        if (std::is_same<S, double>::value)
+            sCg = reinterpret_cast<double4 *>(base + (size_t)m.block_cells * C * 4 * sizeof(V4<S>));
+        stage_records<S>(st_in, grad, std::is_same<S, double>::value ? m.cgeo : nullptr, B, b0, C, c0, ncell, sRec,
+                         sCg);
+        nst = ncell;
+        __syncthreads();
+    }
     const int nwork = (e1 - e0) * C;
     for (int k = threadIdx.x; k < nwork; k += kBlock) {
         const int el = k / C, c = k - el * C, b = b0 + c;
         if (b >= B || a.done[b]) continue;
         S F[4];
-        const SideLoader<S, RecB> ld{st_in, grad, m.cgeo, RecB{B, b}, nullptr, nullptr, 0, 0};
+        const SideLoader<S, RecB> ld{st_in, grad, m.cgeo, RecB{B, b}, sRec, sCg, c0, nst, C, c};
         item_flux<S>(m, prm, e0 + el, ld, a.v3g[b], F);
 #pragma unroll
         for (int q = 0; q < 4; q++) sF[q * stride + el * C + c] = F[q];
@@ -91,8 +107,6 @@ __global__ __launch_bounds__(kBlock) void kb_flux(DevMesh<S> m, DevParams prm, B
     __syncthreads();
     double sq[4] = {0, 0, 0, 0};
     int err = 0;
-    const int c0 = blk * m.block_cells;
-    const int ncell = min(N, c0 + m.block_cells) - c0;
     const size_t NR = (size_t)N * B;
     // kBlock is a multiple of C, so every thread serves the one case c = tid % C
     for (int k = threadIdx.x; k < ncell * C; k += kBlock) {
@@ -246,7 +260,7 @@ struct plbfs_batch {
     bool f32 = false;
     // shared mesh, layout and stream
     plbfs_ctx *base = nullptr;
-    int block_cells = 256, C = 1;
+    int block_cells = 256, C = 1, stage = 0;
     std::vector<void *> allocs;
     // device arrays over N * B records
     void *stA = nullptr, *stB = nullptr, *grad = nullptr, *t1 = nullptr;
@@ -306,13 +320,15 @@ plbfs_mesh_view view_of(plbfs_batch *b) {
                            b->fc.data(), b->fn.data(), b->fa.data(), b->fdt.data()};
 }
 
-size_t lds_bytes_for(const plbfs::Layout &L, int C, bool f32) {
-    const size_t slots = (size_t)L.max_items_per_block * 4 * (f32 ? 4 : 8) * C;
-    return std::max(slots, (size_t)kBlock * 5 * sizeof(double));
+size_t lds_bytes_for(const plbfs::Layout &L, int C, bool f32, bool stage) {
+    const size_t b = f32 ? flux_lds_bytes<float>(L.max_items_per_block, L.block_cells, stage, C)
+                        : flux_lds_bytes<double>(L.max_items_per_block, L.block_cells, stage, C);
+    return std::max(b, (size_t)kBlock * 5 * sizeof(double));
 }
 
-// (Re)builds the shared layout for a flux block of block_cells cells and C cases.
-int retile(plbfs_batch *b, int block_cells, int C) {
+// (Re)builds the shared layout for a flux block of block_cells cells and C
+// cases, staging the tile's records in LDS when stage is set.
+int retile(plbfs_batch *b, int block_cells, int C, int stage) {
     if (block_cells < 64 || block_cells > 1024 || block_cells % 64)
         return plbfs_fail(PLBFS_ERR_INPUT, "block_cells %d: need a multiple of 64 in [64, 1024]", block_cells);
     if (C < 1 || C > kBlock || kBlock % C)
@@ -328,7 +344,11 @@ int retile(plbfs_batch *b, int block_cells, int C) {
     if (rc) return rc;
     b->block_cells = block_cells;
     b->C = C;
-    b->base->lds_bytes = lds_bytes_for(b->base->L, C, b->f32);
+    b->stage = stage;
+    b->base->lds_bytes = lds_bytes_for(b->base->L, C, b->f32, stage != 0);
+    if (b->base->lds_bytes > (size_t)kMaxLds)
+        return plbfs_fail(PLBFS_ERR_INPUT, "tile of %d cells x %d cases needs %zu B of LDS (limit %d)", block_cells,
+                          C, b->base->lds_bytes, kMaxLds);
     const size_t need = (size_t)b->base->L.n_blocks * b->B * 5;
     if (need > b->partials_cap) {
         if ((rc = balloc(b, (void **)&b->partials, need * sizeof(double)))) return rc;
@@ -337,7 +357,7 @@ int retile(plbfs_batch *b, int block_cells, int C) {
     return PLBFS_OK;
 }
 
-BatchArgs args_of(plbfs_batch *b) { return BatchArgs{b->B, b->C, b->d_visc, b->d_v3g, b->done}; }
+BatchArgs args_of(plbfs_batch *b) { return BatchArgs{b->B, b->C, b->stage, b->d_visc, b->d_v3g, b->done}; }
 
 int ibm_upload(plbfs_batch *b) {
     b->ibm_dirty = false;
@@ -584,7 +604,7 @@ extern "C" int plbfs_batch_create(const plbfs_mesh_view *mesh, const plbfs_param
         plbfs_batch_destroy(b);
         return rc;
     };
-    int rc = retile(b, 256, 1);
+    int rc = retile(b, 256, 1, 0);
     if (rc) return fail(rc);
     const size_t NR = (size_t)N * n_cases;
     const size_t sb = b->f32 ? 4 : 8;
@@ -732,17 +752,17 @@ extern "C" int plbfs_batch_run(plbfs_batch *b, int32_t max_steps, plbfs_result *
     return worst;
 }
 
-extern "C" int plbfs_batch_set_tiling(plbfs_batch *b, int32_t block_cells, int32_t cases_per_block) {
+extern "C" int plbfs_batch_set_tiling(plbfs_batch *b, int32_t block_cells, int32_t cases_per_block, int32_t stage) {
     if (!b) return plbfs_fail(PLBFS_ERR_INPUT, "null argument");
     const bool ibm = std::any_of(b->ibm_case.begin(), b->ibm_case.end(), [](const BatchIbmCase &c) { return c.on; });
-    int rc = retile(b, block_cells, cases_per_block);
+    int rc = retile(b, block_cells, cases_per_block, stage ? 1 : 0);
     if (rc) return rc;
     if (ibm) b->ibm_dirty = true;  // the centroid and volume arrays belong to the new layout
     return PLBFS_OK;
 }
 
 extern "C" int plbfs_batch_autotune(plbfs_batch *b, int32_t trial_steps, int32_t *block_cells,
-                                    int32_t *cases_per_block, double *ms_per_step) {
+                                    int32_t *cases_per_block, int32_t *stage, double *ms_per_step) {
     if (!b || trial_steps <= 0) return plbfs_fail(PLBFS_ERR_INPUT, "bad arguments");
     // snapshot what a trial changes: state, RK accumulator start, node positions
     const size_t NR = (size_t)b->N * b->B, sb = b->f32 ? 4 : 8;
@@ -755,45 +775,50 @@ extern "C" int plbfs_batch_autotune(plbfs_batch *b, int32_t trial_steps, int32_t
     const int cand_cells[] = {64, 128, 256, 512};
     const int cand_cases[] = {1, 2, 4, 8, 16, 32};
     double best = 1e300;
-    int bc_best = b->block_cells, c_best = b->C;
+    int bc_best = b->block_cells, c_best = b->C, s_best = b->stage;
     const bool ibm = std::any_of(b->ibm_case.begin(), b->ibm_case.end(), [](const BatchIbmCase &c) { return c.on; });
     for (int bc : cand_cells) {
         for (int C : cand_cases) {
-            if (C > b->B || bc * C < kBlock) continue;  // at least one (cell, case) per thread
-            if ((rc = retile(b, bc, C))) {
-                if (rc == PLBFS_ERR_INPUT) continue;  // does not fit LDS
-                return rc;
+            for (int stage = 0; stage < 2; stage++) {
+                if (C > b->B || bc * C < kBlock) continue;  // at least one (cell, case) per thread
+                if ((rc = retile(b, bc, C, stage))) {
+                    if (rc == PLBFS_ERR_INPUT) continue;  // does not fit LDS
+                    return rc;
+                }
+                if (ibm && (rc = ibm_upload(b))) return rc;
+                float ms = 0;
+                if ((rc = run_steps(b, 1, nullptr)) || (rc = run_steps(b, trial_steps, &ms))) return rc;
+                const double per = ms / trial_steps;
+                if (per < best) {
+                    best = per;
+                    bc_best = bc;
+                    c_best = C;
+                    s_best = stage;
+                }
+                // restore
+                HIP_TRY(hipMemcpy(b->stA, s_st.data(), s_st.size(), hipMemcpyHostToDevice));
+                HIP_TRY(hipMemcpy(b->t0, s_t0.data(), s_t0.size(), hipMemcpyHostToDevice));
+                if ((rc = reset_case_ctl(b))) return rc;
             }
-            if (b->base->lds_bytes > (size_t)kMaxLds) continue;
-            if (ibm && (rc = ibm_upload(b))) return rc;
-            float ms = 0;
-            if ((rc = run_steps(b, 1, nullptr)) || (rc = run_steps(b, trial_steps, &ms))) return rc;
-            const double per = ms / trial_steps;
-            if (per < best) {
-                best = per;
-                bc_best = bc;
-                c_best = C;
-            }
-            // restore
-            HIP_TRY(hipMemcpy(b->stA, s_st.data(), s_st.size(), hipMemcpyHostToDevice));
-            HIP_TRY(hipMemcpy(b->t0, s_t0.data(), s_t0.size(), hipMemcpyHostToDevice));
-            if ((rc = reset_case_ctl(b))) return rc;
         }
     }
-    if ((rc = retile(b, bc_best, c_best))) return rc;
+    if ((rc = retile(b, bc_best, c_best, s_best))) return rc;
     b->ibm_case = keep_ibm[0];
     b->ibm_dirty = ibm;
     if (block_cells) *block_cells = bc_best;
     if (cases_per_block) *cases_per_block = c_best;
+    if (stage) *stage = s_best;
     if (ms_per_step) *ms_per_step = best;
     return PLBFS_OK;
 }
 
-extern "C" int plbfs_batch_layout_info(plbfs_batch *b, int64_t *info4) {
-    if (!b || !info4) return plbfs_fail(PLBFS_ERR_INPUT, "null argument");
-    info4[0] = b->B;
-    info4[1] = b->block_cells;
-    info4[2] = b->C;
-    info4[3] = (int64_t)b->base->lds_bytes;
+extern "C" int plbfs_batch_layout_info(plbfs_batch *b, int64_t *info5) {
+    if (!b || !info5) return plbfs_fail(PLBFS_ERR_INPUT, "null argument");
+    if (!b->base) return plbfs_fail(PLBFS_ERR_INPUT, "batch has no layout");
+    info5[0] = b->B;
+    info5[1] = b->block_cells;
+    info5[2] = b->C;
+    info5[3] = (int64_t)b->base->lds_bytes;
+    info5[4] = b->stage;
     return PLBFS_OK;
 }

@@ -39,22 +39,6 @@ constexpr int kFinalizeThreads = 256;
 // ---------------------------------------------------------------------------
 
 // Byte offset of the staged records behind the face slots in a flux block's LDS.
-template <typename S>
-__host__ __device__ inline size_t staged_offset(int max_items) {
-    return ((size_t)4 * max_items * sizeof(S) + 31) / 32 * 32;
-}
-
-// Dynamic LDS of one flux block: face slots, and with staging the block's own
-// cell records (+ centroid records in FP64 storage).
-template <typename S>
-inline size_t flux_lds_bytes(int max_items, int block_cells, bool stage) {
-    size_t b = staged_offset<S>(max_items);
-    if (stage) {
-        b += (size_t)block_cells * 4 * sizeof(V4<S>);
-        if (std::is_same<S, double>::value) b += (size_t)block_cells * sizeof(double4);
-    }
-    return b;
-}
 
 // Face-item record loader of flux block blk; with staging, first copies the
 // block's own cell records into LDS behind the face slots (then barriers).
@@ -67,6 +51,8 @@ __device__ __forceinline__ SideLoader<S, Rec1> block_loader(const DevMesh<S> &m,
     ld.cgeo = m.cgeo;
     ld.c0 = blk * m.block_cells;
     ld.nc = 0;
+    ld.C = 1;
+    ld.cl = 0;
     ld.sRec = nullptr;
     ld.sCg = nullptr;
     if (m.stage) {
@@ -76,7 +62,7 @@ __device__ __forceinline__ SideLoader<S, Rec1> block_loader(const DevMesh<S> &m,
         double4 *sCg = std::is_same<S, double>::value
                            ? reinterpret_cast<double4 *>(base + (size_t)m.block_cells * 4 * sizeof(V4<S>))
                            : nullptr;
-        stage_records<S>(st, grad, std::is_same<S, double>::value ? m.cgeo : nullptr, Rec1{}, ld.c0, ld.nc, sRec,
+        stage_records<S>(st, grad, std::is_same<S, double>::value ? m.cgeo : nullptr, 1, 0, 1, ld.c0, ld.nc, sRec,
                          sCg);
         ld.sRec = sRec;
         ld.sCg = sCg;

@@ -150,13 +150,14 @@ struct SideLoader {
     const V4<S> *__restrict__ grad;
     const double4 *__restrict__ cgeo;
     R rec;
-    const V4<S> *sRec;
+    const V4<S> *sRec;  // staged records, cell-major then case lane: 4 * (rel * C + cl)
     const double4 *sCg;
     int c0, nc;
+    int C, cl;  // cases staged per cell and this item's case lane (1, 0 for one case)
     __device__ __forceinline__ void side(int c, S p[4], S g[12]) const {
         const unsigned rel = (unsigned)(c - c0);
         if (rel < (unsigned)nc) {
-            const V4<S> *q = sRec + 4 * rel;
+            const V4<S> *q = sRec + 4 * (rel * C + cl);
             unpack_side<S>(opaque4(q[0]), opaque4(q[1]), opaque4(q[2]), opaque4(q[3]), p, g);
         } else {
             const size_t r = rec(c);
@@ -171,19 +172,42 @@ struct SideLoader {
     }
 };
 
-// Copies the records of cells [c0, c0 + nc) of one case into LDS (4 V4 per
-// cell) and, when cgeo is given, their centroid records.
-template <typename S, typename R>
+// Copies the records of cells [c0, c0 + nc) into LDS, 4 V4 per (cell, case
+// lane) for the C cases b0 .. b0 + C - 1 of a batch of B (C = B = 1 for one
+// case; lanes past the last case are left unset), and, when cgeo is given,
+// the cells' centroid records.
+template <typename S>
 __device__ __forceinline__ void stage_records(const V4<S> *__restrict__ st, const V4<S> *__restrict__ grad,
-                                              const double4 *__restrict__ cgeo, R rec, int c0, int nc,
-                                              V4<S> *sRec, double4 *sCg) {
-    for (int k = threadIdx.x; k < nc * 4; k += blockDim.x) {
+                                              const double4 *__restrict__ cgeo, int B, int b0, int C, int c0,
+                                              int nc, V4<S> *sRec, double4 *sCg) {
+    for (int k = threadIdx.x; k < nc * C * 4; k += blockDim.x) {
         const int r = k >> 2, q = k & 3;
-        const size_t g = rec(c0 + r);
+        const int il = r / C, cl = r - il * C;
+        if (b0 + cl >= B) continue;
+        const size_t g = (size_t)(c0 + il) * B + b0 + cl;
         sRec[k] = q == 0 ? st[g] : grad[3 * g + q - 1];
     }
     if (cgeo)
         for (int k = threadIdx.x; k < nc; k += blockDim.x) sCg[k] = cgeo[c0 + k];
+}
+
+// Byte offset of the staged records behind the face slots (C cases per slot)
+// in a flux block's LDS.
+template <typename S>
+__host__ __device__ inline size_t staged_offset(int max_items, int C = 1) {
+    return ((size_t)4 * max_items * C * sizeof(S) + 31) / 32 * 32;
+}
+
+// Dynamic LDS of one flux block: face slots, and with staging the block's own
+// cell records for C cases (+ centroid records in FP64 storage).
+template <typename S>
+inline size_t flux_lds_bytes(int max_items, int block_cells, bool stage, int C = 1) {
+    size_t b = staged_offset<S>(max_items, C);
+    if (stage) {
+        b += (size_t)block_cells * C * 4 * sizeof(V4<S>);
+        if (std::is_same<S, double>::value) b += (size_t)block_cells * sizeof(double4);
+    }
+    return b;
 }
 
 // Flux of face work item e (src/Solver.cpp:286-347 with populate_lattice_macros

@@ -131,8 +131,8 @@ def lib():
     L.plbfs_batch_get_state.argtypes = [vp, C.c_int32, dp, dp, dp, dp]
     L.plbfs_batch_set_ibm.argtypes = [vp, C.c_int32, C.POINTER(IbmNodes)]
     L.plbfs_batch_run.argtypes = [vp, C.c_int32, C.POINTER(Result)]
-    L.plbfs_batch_set_tiling.argtypes = [vp, C.c_int32, C.c_int32]
-    L.plbfs_batch_autotune.argtypes = [vp, C.c_int32, ip, ip, dp]
+    L.plbfs_batch_set_tiling.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32]
+    L.plbfs_batch_autotune.argtypes = [vp, C.c_int32, ip, ip, ip, dp]
     L.plbfs_batch_layout_info.argtypes = [vp, i64p]
     fp = C.POINTER(C.c_float)
     L.plbfs_mpnn_n_params.restype = C.c_int

@@ -270,17 +270,20 @@ class GpuBatch:
         return rc, [dict(status=r.status, steps=r.steps, converged=bool(r.converged), residual=list(r.residual),
                          ms=r.solve_ms) for r in res]
 
-    def set_tiling(self, block_cells, cases_per_block):
-        nat.check(nat.lib().plbfs_batch_set_tiling(self.h, int(block_cells), int(cases_per_block)))
+    def set_tiling(self, block_cells, cases_per_block, stage=0):
+        nat.check(nat.lib().plbfs_batch_set_tiling(self.h, int(block_cells), int(cases_per_block), int(stage)))
 
     def autotune(self, trial_steps=5):
+        """(cells per block, cases per workgroup, staging, ms per step) of the fastest tiling."""
         bc = C.c_int32(0)
         cp = C.c_int32(0)
+        sg = C.c_int32(0)
         ms = C.c_double(0)
-        nat.check(nat.lib().plbfs_batch_autotune(self.h, int(trial_steps), C.byref(bc), C.byref(cp), C.byref(ms)))
-        return bc.value, cp.value, ms.value
+        nat.check(nat.lib().plbfs_batch_autotune(self.h, int(trial_steps), C.byref(bc), C.byref(cp), C.byref(sg),
+                                                 C.byref(ms)))
+        return bc.value, cp.value, sg.value, ms.value
 
     def layout_info(self):
-        info = np.zeros(4, dtype=np.int64)
+        info = np.zeros(5, dtype=np.int64)
         nat.check(nat.lib().plbfs_batch_layout_info(self.h, info.ctypes.data_as(C.POINTER(C.c_int64))))
-        return dict(zip(["cases", "block_cells", "cases_per_block", "lds_bytes"], (int(x) for x in info)))
+        return dict(zip(["cases", "block_cells", "cases_per_block", "lds_bytes", "stage"], (int(x) for x in info)))

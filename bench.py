@@ -120,9 +120,9 @@ def run_batch(args):
         bt.set_ibm(b, **ws.ibm_args(d, c))
         nodes += d.n_nodes
     t = time.perf_counter()
-    bc, cp, tune_ms = bt.autotune(3)
-    print("autotune: block_cells %d cases_per_block %d (%.3f ms/step, %.1f s)" % (bc, cp, tune_ms,
-          time.perf_counter() - t), file=sys.stderr, flush=True)
+    bc, cp, sg, tune_ms = bt.autotune(3)
+    print("autotune: block_cells %d cases_per_block %d stage %d (%.3f ms/step, %.1f s)" % (
+        bc, cp, sg, tune_ms, time.perf_counter() - t), file=sys.stderr, flush=True)
     if args.warmup > 0:
         bt.run(args.warmup)
     rc, res = bt.run(args.steps)
@@ -143,7 +143,8 @@ def run_batch(args):
         "data": "synthetic mesh (seeded), seeded case draws, uniform-inflow initial state",
         "config": {"workload": "%d cases on a jittered 64x32x32 hex channel (65,536 cells), Re~U[20,200], "
                                "sphere D~U[6,14] cells (%d Lagrangian nodes in total), Peskin-4" % (B, nodes),
-                   "cases": B, "cells": N, "faces": F, "block_cells": bc, "cases_per_block": cp},
+                   "cases": B, "cells": N, "faces": F, "block_cells": bc, "cases_per_block": cp,
+                   "staged": bool(sg)},
         "roofline": {"bound": "hbm", "achieved": step_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": step_gbs / HBM_PEAK_GBS, "traffic": None, "kernel": "whole step (graph replay)",
                      "alg_bytes_per_case_cell_update": b_update},

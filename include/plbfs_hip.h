@@ -225,15 +225,16 @@ int plbfs_batch_set_ibm(plbfs_batch *batch, int32_t case_id, const plbfs_ibm_nod
 /* results[n_cases]: per-case status, steps, convergence and residual; the
  * return value is the first failing case's status, or PLBFS_OK. */
 int plbfs_batch_run(plbfs_batch *batch, int32_t max_steps, plbfs_result *results);
-/* Tiling: cells per flux block (multiple of 64, 64..1024) and cases per
- * workgroup (a divisor of 256). */
-int plbfs_batch_set_tiling(plbfs_batch *batch, int32_t block_cells, int32_t cases_per_block);
+/* Tiling: cells per flux block (multiple of 64, 64..1024), cases per
+ * workgroup (a divisor of 256), and whether the tile stages its own (cell,
+ * case) records in LDS. */
+int plbfs_batch_set_tiling(plbfs_batch *batch, int32_t block_cells, int32_t cases_per_block, int32_t stage);
 /* Times trial_steps steps of every tiling that fits the LDS, keeps the
  * fastest, and restores the state and node positions set before the call. */
 int plbfs_batch_autotune(plbfs_batch *batch, int32_t trial_steps, int32_t *block_cells, int32_t *cases_per_block,
-                         double *ms_per_step);
-/* info[0..3]: cases, cells per flux block, cases per workgroup, LDS bytes. */
-int plbfs_batch_layout_info(plbfs_batch *batch, int64_t *info4);
+                         int32_t *stage, double *ms_per_step);
+/* info[0..4]: cases, cells per flux block, cases per workgroup, LDS bytes, staging. */
+int plbfs_batch_layout_info(plbfs_batch *batch, int64_t *info5);
 
 #define PLBFS_KSTAT_GRAD 0
 #define PLBFS_KSTAT_FLUX 1

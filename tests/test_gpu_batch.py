@@ -81,7 +81,7 @@ def test_batch_fp64_matches_oracle_per_case(gpu_available):
         assert max(errs) < 1e-10, (b, errs)
 
 
-@pytest.mark.parametrize("tiling", [(64, 4), (128, 2), (256, 1), (512, 1), (64, 8)])
+@pytest.mark.parametrize("tiling", [(64, 4, 0), (128, 2, 1), (256, 1, 1), (512, 1, 0), (64, 8, 1), (128, 4, 1)])
 def test_batch_tiling_does_not_change_fields(gpu_available, tiling):
     pm = channel_mesh(12, 6, 6, tets=True)
     res = [25.0, 50.0, 75.0, 100.0, 150.0, 200.0, 40.0, 80.0]
@@ -91,10 +91,11 @@ def test_batch_tiling_does_not_change_fields(gpu_available, tiling):
     o = oracle_for(pm, cases[0])
     st = perturbed_state(cases[0], o, amp=0.01, seed=8)
     outs = []
-    for t in [(256, 1), tiling]:
+    for t in [(256, 1, 0), tiling]:
         hm, bt = make_batch(pm, cases)
         bt.set_tiling(*t)
-        assert bt.layout_info()["block_cells"] == t[0]
+        info = bt.layout_info()
+        assert (info["block_cells"], info["cases_per_block"], info["stage"]) == t
         bt.set_state(-1, *st)
         assert bt.run(25)[0] == 0
         outs.append([bt.state(b) for b in range(len(cases))])
@@ -154,10 +155,10 @@ def test_autotune_keeps_results(gpu_available):
     st = perturbed_state(cases[0], o, amp=0.01, seed=5)
     hm, bt = make_batch(pm, cases)
     bt.set_state(-1, *st)
-    bc, cp, ms = bt.autotune(3)
-    assert bc in (64, 128, 256, 512) and cp in (1, 2, 4) and ms > 0
+    bc, cp, sg, ms = bt.autotune(3)
+    assert bc in (64, 128, 256, 512) and cp in (1, 2, 4) and sg in (0, 1) and ms > 0
     info = bt.layout_info()
-    assert (info["block_cells"], info["cases_per_block"]) == (bc, cp)
+    assert (info["block_cells"], info["cases_per_block"], info["stage"]) == (bc, cp, sg)
     assert bt.run(20)[0] == 0
     ref = GpuBatch(hm, cases[0].params, cases[0].patches_for(pm.patch_names), [c.visc for c in cases])
     ref.set_state(-1, *st)
