@@ -11,6 +11,7 @@
 #include <sys/stat.h>
 
 #include <chrono>
+#include <fstream>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -36,7 +37,7 @@ struct BcSpec {
 };
 
 struct ObjSpec {
-    std::string name, type;
+    std::string name, type, mesh_file;
     int num_nodes = 0;
     double radius = 0, depth = 0, stiffness = 0, cx = 0, cy = 0, cz = 0;
     int kernel = PLBFS_DELTA_HAT2;
@@ -163,6 +164,7 @@ extern "C" int plbfs_case_load(const char *xml_path, plbfs_case **out) {
             o.cx = std::atof(t("centre_x").c_str());
             o.cy = std::atof(t("centre_y").c_str());
             o.cz = std::atof(t("centre_z").c_str());
+            o.mesh_file = t("mesh_file");
             const std::string k = t("delta");
             o.kernel = (k == "peskin4" || (k.empty() && o.type == "rigid_sphere")) ? PLBFS_DELTA_PESKIN4
                                                                                     : PLBFS_DELTA_HAT2;
@@ -314,6 +316,36 @@ extern "C" int plbfs_case_object_nodes(const plbfs_case *c, int32_t k, double *x
             pts.push_back(o.cz + o.radius * z);
         }
         area = 4 * PI * o.radius * o.radius / n;
+    } else if (o.type == "spring_network") {
+        // lagrangian_object::mesh_read (lagrangian_object.cpp:116-213): node,
+        // spring and triangle counts, node coordinates (used as given), then
+        // triangle and spring connectivity and per-node neighbour lists, all
+        // 1-based.  The solver moves every node on its own anchor spring like
+        // the rigid objects; the area per node is the reference's cylinder
+        // expression with the file's node count (immersed_boundary_method.cu:168).
+        std::ifstream in(o.mesh_file);
+        if (!in) return plbfs_fail(PLBFS_ERR_INPUT, "object '%s': cannot open mesh_file '%s'", o.name.c_str(),
+                                   o.mesh_file.c_str());
+        long nn = -1, ns = -1, nt = -1;
+        in >> nn >> ns >> nt;
+        if (!in || nn <= 0 || ns < 0 || nt < 0)
+            return plbfs_fail(PLBFS_ERR_INPUT, "object '%s': bad counts in '%s'", o.name.c_str(), o.mesh_file.c_str());
+        pts.resize((size_t)3 * nn);
+        for (long i = 0; i < 3 * nn; i++) in >> pts[i];
+        long idx = 0;
+        for (long i = 0; i < 3 * nt + 4 * ns && in; i++) {
+            in >> idx;
+            if (idx < 1 || idx > nn)
+                return plbfs_fail(PLBFS_ERR_INPUT, "object '%s': connectivity index %ld out of 1..%ld",
+                                  o.name.c_str(), idx, nn);
+        }
+        for (long i = 0; i < nn && in; i++) {
+            long deg = 0;
+            in >> deg;
+            for (long j = 0; j < deg && in; j++) in >> idx;
+        }
+        if (!in) return plbfs_fail(PLBFS_ERR_INPUT, "object '%s': '%s' ends early", o.name.c_str(), o.mesh_file.c_str());
+        area = 2 * PI * o.radius / nn * o.depth;
     } else {
         return plbfs_fail(PLBFS_ERR_INPUT, "object '%s': unsupported type '%s'", o.name.c_str(), o.type.c_str());
     }

@@ -231,3 +231,39 @@ def test_create_without_device_is_a_clean_error():
     with pytest.raises(nat.PlbfsError) as e:
         GpuSolver(hm, case.params, case.patches_for(pm.patch_names))
     assert e.value.rc == -6
+
+
+def test_spring_network_object_nodes(tmp_path):
+    """A spring_network object's nodes come from its mesh file as written
+    (lagrangian_object.cpp:116-213); node area uses the reference's cylinder
+    expression with the file's node count; bad connectivity is a clean error."""
+    nodes = np.array([[1.0, 2.0, 3.0], [1.5, 2.0, 3.0], [1.0, 2.5, 3.0], [1.0, 2.0, 3.5]])
+    mesh = tmp_path / "net.txt"
+    lines = ["4 2 2"] + ["%.17g %.17g %.17g" % tuple(p) for p in nodes]
+    lines += ["1 2 3", "1 3 4", "1 2 3 4", "2 3 4 1", "2 2 3", "2 1 3", "2 1 2", "1 1"]
+    mesh.write_text("\n".join(lines) + "\n")
+    obj = ("<lagrangian_objects><object><name>net</name><num_nodes>999</num_nodes><radius>0.5</radius>"
+           "<depth>2</depth><stiffness>3</stiffness><centre_x>0</centre_x><centre_y>0</centre_y>"
+           "<centre_z>0</centre_z><type>spring_network</type><mesh_file>%s</mesh_file></object>"
+           "</lagrangian_objects>\n</root>" % mesh)
+    xml = tmp_path / "case.xml"
+    xml.write_text(CASE_XML.format(out=str(tmp_path / "out"), mesh="unused/", prec="false").replace("</root>", obj))
+    L = nat.lib()
+    h = load_case(str(xml))
+    L.plbfs_case_object_nodes.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_double), C.c_int32,
+                                          C.POINTER(C.c_int32), C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                          C.POINTER(C.c_int32)]
+    assert L.plbfs_case_n_objects(h) == 1
+    x = np.zeros(12)
+    n, k = C.c_int32(0), C.c_int32(0)
+    st, area = C.c_double(0), C.c_double(0)
+    nat.check(L.plbfs_case_object_nodes(h, 0, nat.dptr(x), 4, C.byref(n), C.byref(st), C.byref(area), C.byref(k)))
+    assert n.value == 4 and st.value == 3.0
+    assert np.array_equal(x.reshape(4, 3), nodes)
+    assert area.value == pytest.approx(2 * np.pi * 0.5 / 4 * 2.0, rel=1e-15)
+    L.plbfs_case_free(h)
+    mesh.write_text("\n".join(lines).replace("1 2 3 4", "1 2 3 9") + "\n")
+    h = load_case(str(xml))
+    rc = L.plbfs_case_object_nodes(h, 0, nat.dptr(x), 4, C.byref(n), C.byref(st), C.byref(area), C.byref(k))
+    assert rc == -1
+    L.plbfs_case_free(h)
