@@ -122,9 +122,11 @@ struct TimedLaunch {
 };
 
 // plbfs_create with an explicit flux block size (64..1024 cells, a multiple of
-// 64) and LDS sized for lds_cases cases per block (the batched solver).
+// 64), LDS sized for lds_cases cases per block (the batched solver sizes its
+// own), and staging of the flux block's own records.
 int create_ctx(const plbfs_mesh_view *mesh, const plbfs_params *params, const plbfs_patch *patches,
-               int32_t n_patches, const int32_t *bface_patch, int block_cells, int lds_cases, plbfs_ctx **out);
+               int32_t n_patches, const int32_t *bface_patch, int block_cells, int lds_cases, bool flux_stage,
+               plbfs_ctx **out);
 
 // Launches the per-step immersed-boundary kernels (ibm.hip); no-op when off.
 int ibm_launch(plbfs_ctx *c);

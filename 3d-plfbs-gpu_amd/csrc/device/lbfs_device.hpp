@@ -64,6 +64,7 @@ struct DevMesh {
     int xcd_remap;
     int block_cells;  // cells per flux block (a multiple of 64)
     int stage;        // 1: flux blocks stage their own cells' records in LDS
+    int grad_stage;   // 1: gradient blocks stage their own cells' states in LDS
     int64_t E;
     const uint8_t *st_cnt;
     const int32_t *st_nbr;

@@ -80,7 +80,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
     const int c0 = xcd_block(blockIdx.x, gridDim.x, m.xcd_remap) * kBlock;
     const int i = c0 + threadIdx.x;
     S g[12];
-    if (m.stage) {
+    if (m.grad_stage) {
         // the block's own states in LDS: most stencil neighbours are in the block
         __shared__ V4<S> sSt[kBlock];
         const int nc = min(m.N - c0, kBlock);
@@ -316,26 +316,31 @@ extern "C" int plbfs_device_count(void) {
     return n;
 }
 
-static bool stage_default() {
-    const char *sg = std::getenv("PLBFS_STAGE");
-    return !(sg && sg[0] == '0');
+static bool env_flag(const char *name, bool def) {
+    const char *v = std::getenv(name);
+    if (!v || !v[0]) return def;
+    return v[0] != '0';
 }
 
 extern "C" int plbfs_create(const plbfs_mesh_view *mesh, const plbfs_params *params, const plbfs_patch *patches,
                             int32_t n_patches, const int32_t *bface_patch, plbfs_ctx **out) {
     // cells per flux block (a multiple of 64): larger blocks duplicate fewer
-    // border faces but need more LDS; with staged records FP64 blocks hold 192
-    // cells to stay within the 64 KiB of one workgroup
+    // border faces but need more LDS
     int block_cells = kFluxBlockCells;
-    if (params && params->precision == PLBFS_FP64 && stage_default()) block_cells = 192;
     if (const char *bc = std::getenv("PLBFS_BLOCK_CELLS")) block_cells = std::atoi(bc);
     if (block_cells < 64 || block_cells % 64 || block_cells > 4 * kBlock)
         return plbfs_fail(PLBFS_ERR_INPUT, "PLBFS_BLOCK_CELLS must be a multiple of 64 up to %d", 4 * kBlock);
-    return create_ctx(mesh, params, patches, n_patches, bface_patch, block_cells, 1, out);
+    // measured on MI355X (4.09M tets): staging the flux block's own records in
+    // LDS costs more (staging phase + barrier) than the L2 traffic it saves,
+    // while staging the gradient kernel's 256 states saves 12 %; flux staging
+    // is therefore opt-in (PLBFS_FLUX_STAGE=1) and gradient staging opt-out
+    const bool flux_stage = env_flag("PLBFS_FLUX_STAGE", false);
+    return create_ctx(mesh, params, patches, n_patches, bface_patch, block_cells, 1, flux_stage, out);
 }
 
 int create_ctx(const plbfs_mesh_view *mesh, const plbfs_params *params, const plbfs_patch *patches,
-               int32_t n_patches, const int32_t *bface_patch, int block_cells, int lds_cases, plbfs_ctx **out) {
+               int32_t n_patches, const int32_t *bface_patch, int block_cells, int lds_cases, bool flux_stage,
+               plbfs_ctx **out) {
     if (!mesh || !params || !out) return plbfs_fail(PLBFS_ERR_INPUT, "null argument");
     *out = nullptr;
     const plbfs_params &p = *params;
@@ -357,8 +362,7 @@ int create_ctx(const plbfs_mesh_view *mesh, const plbfs_params *params, const pl
     }
     const Layout &L = c->L;
     const size_t sbytes = c->f32 ? sizeof(float) : sizeof(double);
-    // staging the block's own cell records in LDS (single-case blocks)
-    const bool stage = lds_cases == 1 && stage_default();
+    const bool stage = flux_stage;
     c->lds_bytes = lds_cases == 1
                        ? (c->f32 ? flux_lds_bytes<float>(L.max_items_per_block, block_cells, stage)
                                  : flux_lds_bytes<double>(L.max_items_per_block, block_cells, stage))
@@ -385,6 +389,7 @@ int create_ctx(const plbfs_mesh_view *mesh, const plbfs_params *params, const pl
     d.E = L.n_items;
     d.block_cells = L.block_cells;
     d.stage = stage ? 1 : 0;
+    d.grad_stage = env_flag("PLBFS_GRAD_STAGE", true) ? 1 : 0;
     d.g0x = L.ghost0[0];
     d.g0y = L.ghost0[1];
     d.g0z = L.ghost0[2];
@@ -441,6 +446,7 @@ int create_ctx(const plbfs_mesh_view *mesh, const plbfs_params *params, const pl
         e.xcd_remap = d.xcd_remap;
         e.block_cells = d.block_cells;
         e.stage = d.stage;
+        e.grad_stage = d.grad_stage;
         e.it_fc = nullptr;
         e.cgeo = nullptr;
         e.g0x = d.g0x; e.g0y = d.g0y; e.g0z = d.g0z;

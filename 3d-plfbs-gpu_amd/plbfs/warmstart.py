@@ -13,6 +13,7 @@ Pipeline (all solver and network work on the GPU through the C-ABI):
 from __future__ import annotations
 
 import math
+import sys
 from dataclasses import dataclass
 
 import numpy as np
@@ -78,7 +79,16 @@ def generate(pm, hm, draws, max_steps, tiling=None, precision=1):
     bt.set_state(-1, *uniform_state(cases[0], geo, N))
     for b, (d, c) in enumerate(zip(draws, cases)):
         bt.set_ibm(b, **ibm_args(d, c))
-    rc, res = bt.run(max_steps)
+    done = 0
+    while done < max_steps:
+        n = min(5000, max_steps - done)
+        rc, res = bt.run(n)
+        done += n
+        live = sum(1 for r in res if not r["converged"] and r["status"] == 0)
+        print("  batch of %d: %d steps, %d cases still running" % (len(draws), done, live), file=sys.stderr,
+              flush=True)
+        if live == 0:
+            break
     fields = [bt.state(b) for b in range(len(draws))]
     bt.close()
     return fields, res
@@ -124,8 +134,14 @@ def solve_to_tolerance(pm, hm, case, state, ibm, max_steps):
     s = GpuSolver(hm, case.params, case.patches_for(pm.patch_names))
     s.set_state(*state)
     s.set_ibm(**ibm)
-    r = s.run(max_steps)
-    out = dict(rc=r["rc"], steps=r["steps"], converged=r["converged"], ms=r["ms"], residual=r["residual"])
+    steps, ms = 0, 0.0
+    while steps < max_steps:
+        r = s.run(min(5000, max_steps - steps))
+        steps += r["steps"]
+        ms += r["ms"]
+        if r["converged"] or r["rc"] != 0:
+            break
+    out = dict(rc=r["rc"], steps=steps, converged=r["converged"], ms=ms, residual=r["residual"])
     st = s.state()
     s.close()
     return out, st

@@ -175,10 +175,21 @@ def run_to_tolerance(args):
             X, dA = objects.fibonacci_sphere(ibm["n_nodes"], ibm["centre"], ibm["radius"])
             s.set_ibm(X, X, stiffness=ibm["stiffness"], node_area=dA, dt=case.time_marching_step, h=ibm["h"],
                       kernel=ibm["kernel"])
+        # chunks of 5000 steps, so that progress is printed while a long solve runs
         t = time.perf_counter()
-        r = s.run(args.max_steps)
+        hist, dev_ms, steps = [], 0.0, 0
+        while steps < args.max_steps:
+            r = s.run(min(5000, args.max_steps - steps))
+            hist.append(r["hist"])
+            dev_ms += r["ms"]
+            steps += r["steps"]
+            print("  %s: %d steps, residual %.3e" % (prec, steps, max(r["residual"][1:])), file=sys.stderr,
+                  flush=True)
+            if r["converged"] or r["rc"] != 0:
+                break
         wall = time.perf_counter() - t
-        h = r["hist"]
+        r = dict(r, steps=steps, ms=dev_ms)
+        h = np.concatenate(hist)
         marks = {str(k): float(max(h[k - 1, 1:])) for k in (1, 100, 1000, 5000, 10000, 20000, 40000, 80000)
                  if k <= len(h)}
         out[prec] = dict(steps=r["steps"], converged=r["converged"], rc=r["rc"], seconds=wall,

@@ -49,7 +49,8 @@ def test_fp32_staging_is_bitwise_neutral(gpu_available, monkeypatch, stage):
     st = perturbed_state(case, o, amp=0.01, seed=12)
     outs = []
     for stg in (stage, 1 - stage):
-        monkeypatch.setenv("PLBFS_STAGE", str(stg))
+        monkeypatch.setenv("PLBFS_FLUX_STAGE", str(stg))
+        monkeypatch.setenv("PLBFS_GRAD_STAGE", str(1 - stg))
         s = solver(pm, case, 1)
         s.set_state(*st)
         assert s.run(30)["rc"] == 0

@@ -112,7 +112,7 @@ def test_periodic_z_steps(gpu_available, tets):
 def test_flux_block_too_large_for_lds_is_a_clean_error(gpu_available, monkeypatch):
     # 1024 FP64 tets need ~2300 face slots x 32 B > 64 KiB of LDS
     monkeypatch.setenv("PLBFS_BLOCK_CELLS", "1024")
-    monkeypatch.setenv("PLBFS_STAGE", "0")
+    monkeypatch.setenv("PLBFS_FLUX_STAGE", "0")
     pm = channel_mesh(16, 8, 8, tets=True)
     case = pc.channel_case(re=40.0, length_scale=6.0)
     hm = HostMesh.from_polymesh(pm, case.visc, case.gamma, case.fneq_min)
@@ -128,7 +128,8 @@ def test_flux_block_sizes(gpu_available, monkeypatch, block_cells, stage):
     # reads its own cells' records (LDS or global memory); none changes the
     # arithmetic
     monkeypatch.setenv("PLBFS_BLOCK_CELLS", str(block_cells))
-    monkeypatch.setenv("PLBFS_STAGE", str(stage))
+    monkeypatch.setenv("PLBFS_FLUX_STAGE", str(stage))
+    monkeypatch.setenv("PLBFS_GRAD_STAGE", str(stage))
     pm = channel_mesh(16, 8, 8, tets=True)
     case = pc.channel_case(re=40.0, length_scale=6.0)
     case.lsq_exact_det = 1
