@@ -70,13 +70,16 @@ struct DevMesh {
     const int32_t *st_nbr;
     const S *st_w;
     const double *vol, *inv_vol, *cfl_area;
-    const int32_t *blk_item_off, *it_own, *it_nbr;
-    const uint32_t *it_side;  // per item: lattice side bits (see lattice_pair)
-    // FP64 storage: [5][E] n, A, dt plus it_fc [3][E] face centroid; the
-    // centroid-to-face offsets are formed on the fly from cgeo (same bits as
-    // the host's cc - fc).  FP32 storage: [11][E] n, A, dt, r_owner, r_neighbour.
-    const S *it_geo;
-    const double *it_fc;
+    const int32_t *blk_item_off;
+    // Face work items, one record each so that an item is a few wide loads
+    // from one address.  it_x[e] = (owner, neighbour or -1-b, lattice side bits
+    // (see lattice_pair), 0).  FP64 storage: it_d[2e..2e+1] = (n, A), (dt, x_face);
+    // the centroid-to-face offsets are formed on the fly from cgeo (same bits as
+    // the host's cc - fc).  FP32 storage: it_f[3e..3e+2] = (n, A), (dt, r_owner),
+    // (r_neighbour, 0).
+    const int4 *it_x;
+    const double4 *it_d;
+    const float4 *it_f;
     const double4 *cgeo;  // FP64 storage: per cell (cx, cy, cz, 1/V)
     double g0x, g0y, g0z;  // centroid of the ghost of boundary face 0
     const int32_t *g_off, *g_ent;

@@ -659,20 +659,32 @@ extern "C" int plbfs_mpnn_train(plbfs_mpnn *n, int32_t n_samples, const float *X
         st ^= st << 17;
         return st;
     };
+    auto release = [&]() {
+        for (float *p : {dX, dT}) {
+            n->allocs.erase(std::find(n->allocs.begin(), n->allocs.end(), (void *)p));
+            hipFree(p);
+        }
+    };
     for (int ep = 0; ep < epochs; ep++) {
         for (int i = n_samples - 1; i > 0; i--) std::swap(order[i], order[rnd() % (uint64_t)(i + 1)]);
+        const double lr_min = lr / 20.0, ph = epochs > 1 ? (double)ep / (epochs - 1) : 0.0;
+        const float lr_ep = (float)(lr_min + 0.5 * (lr - lr_min) * (1.0 + std::cos(M_PI * ph)));
         double sum = 0;
         for (int k = 0; k < n_samples; k++) {
             const int sidx = order[k];
             HIP_TRY(hipMemcpyAsync(n->x, dX + nx * sidx, nx * sizeof(float), hipMemcpyDeviceToDevice, n->stream));
             HIP_TRY(hipMemcpyAsync(n->tgt, dT + ny * sidx, ny * sizeof(float), hipMemcpyDeviceToDevice, n->stream));
             double l = 0;
-            if ((rc = loss_grad_device(n, &l))) return rc;
-            if ((rc = plbfs_mpnn_adam_step(n, lr, 0.9f, 0.999f, 1e-8f))) return rc;
+            if ((rc = loss_grad_device(n, &l)) || (rc = plbfs_mpnn_adam_step(n, lr_ep, 0.9f, 0.999f, 1e-8f))) {
+                hipStreamSynchronize(n->stream);
+                release();
+                return rc;
+            }
             sum += l;
         }
         if (loss_hist) loss_hist[ep] = sum / n_samples;
     }
     HIP_TRY(hipStreamSynchronize(n->stream));
+    release();
     return PLBFS_OK;
 }

@@ -71,6 +71,13 @@ __device__ __forceinline__ SideLoader<S, Rec1> block_loader(const DevMesh<S> &m,
     return ld;
 }
 
+// Item index records (owner, neighbour, side bits, 0).
+static std::vector<int4> item_idx(const Layout &L) {
+    std::vector<int4> v((size_t)L.n_items);
+    for (size_t e = 0; e < v.size(); e++) v[e] = int4{L.it_own[e], L.it_nbr[e], (int)L.it_side[e], 0};
+    return v;
+}
+
 // Occupancy 8 waves/SIMD (<= 64 VGPRs) keeps enough gathers in flight; the
 // compiler otherwise settles at 92 VGPRs (5 waves) for the FP64 instance.
 template <typename S>
@@ -420,22 +427,27 @@ int create_ctx(const plbfs_mesh_view *mesh, const plbfs_params *params, const pl
         (rc = upload(c, (double **)&d.vol, L.vol)) || (rc = upload(c, (double **)&d.inv_vol, L.inv_vol)) ||
         (rc = upload(c, (double **)&d.cfl_area, L.cfl_area)) ||
         (rc = upload(c, (int32_t **)&d.blk_item_off, L.blk_item_off)) ||
-        (rc = upload(c, (int32_t **)&d.it_own, L.it_own)) || (rc = upload(c, (int32_t **)&d.it_nbr, L.it_nbr)) ||
-        (rc = upload(c, (uint32_t **)&d.it_side, L.it_side)) ||
+        (rc = upload(c, (int4 **)&d.it_x, item_idx(L))) ||
         (rc = upload(c, (int32_t **)&d.g_off, L.g_off)) || (rc = upload(c, (int32_t **)&d.g_ent, L.g_ent)) ||
         (rc = upload(c, (int32_t **)&d.b_patch, bpatch)) || (rc = upload(c, (int32_t **)&d.b_partner, bpartner)) ||
         (rc = upload(c, (DevPatch **)&d.patches, dp)))
         return fail(rc);
     if (!c->f32) {
         // n, A, dt only; offsets come from the face and cell centroids
-        std::vector<double> geo5(L.it_geo.begin(), L.it_geo.begin() + (size_t)5 * L.n_items);
+        const size_t E = (size_t)L.n_items;
+        std::vector<double4> geo(2 * E);
+        for (size_t e = 0; e < E; e++) {
+            const double *g = L.it_geo.data();
+            geo[2 * e] = double4{g[e], g[E + e], g[2 * E + e], g[3 * E + e]};
+            geo[2 * e + 1] = double4{g[4 * E + e], L.it_fc[e], L.it_fc[E + e], L.it_fc[2 * E + e]};
+        }
         std::vector<double> cg((size_t)4 * N);
         for (int n = 0; n < N; n++) {
             for (int a = 0; a < 3; a++) cg[(size_t)4 * n + a] = L.centroid[(size_t)a * N + n];
             cg[(size_t)4 * n + 3] = L.inv_vol[n];
         }
-        if ((rc = upload(c, (double **)&d.st_w, L.st_w)) || (rc = upload(c, (double **)&d.it_geo, geo5)) ||
-            (rc = upload(c, (double **)&d.it_fc, L.it_fc)) || (rc = upload(c, (double **)&d.cgeo, cg)) ||
+        if ((rc = upload(c, (double **)&d.st_w, L.st_w)) || (rc = upload(c, (double4 **)&d.it_d, geo)) ||
+            (rc = upload(c, (double **)&d.cgeo, cg)) ||
             (rc = upload(c, (double **)&d.b_dmag, bdmag)))
             return fail(rc);
     } else {
@@ -447,17 +459,25 @@ int create_ctx(const plbfs_mesh_view *mesh, const plbfs_params *params, const pl
         e.block_cells = d.block_cells;
         e.stage = d.stage;
         e.grad_stage = d.grad_stage;
-        e.it_fc = nullptr;
+        e.it_d = nullptr;
         e.cgeo = nullptr;
         e.g0x = d.g0x; e.g0y = d.g0y; e.g0z = d.g0z;
         e.st_cnt = d.st_cnt; e.st_nbr = d.st_nbr; e.vol = d.vol; e.inv_vol = d.inv_vol; e.cfl_area = d.cfl_area;
-        e.blk_item_off = d.blk_item_off; e.it_own = d.it_own; e.it_nbr = d.it_nbr; e.g_off = d.g_off;
-        e.it_side = d.it_side;
+        e.blk_item_off = d.blk_item_off; e.it_x = d.it_x; e.g_off = d.g_off;
         e.g_ent = d.g_ent; e.b_patch = d.b_patch; e.b_partner = d.b_partner;
         auto to32 = [](const std::vector<double> &v) { return std::vector<float>(v.begin(), v.end()); };
         std::vector<DevPatch> dp32 = dp;
         for (auto &q : dp32) q.rho -= p.ref_rho;
-        if ((rc = upload(c, (float **)&e.st_w, to32(L.st_w))) || (rc = upload(c, (float **)&e.it_geo, to32(L.it_geo))) ||
+        const size_t E = (size_t)L.n_items;
+        std::vector<float4> geo(3 * E);
+        for (size_t it = 0; it < E; it++) {
+            float g[12] = {0};
+            for (int q = 0; q < kGeoPerItem; q++) g[q] = (float)L.it_geo[(size_t)q * E + it];
+            geo[3 * it] = float4{g[0], g[1], g[2], g[3]};
+            geo[3 * it + 1] = float4{g[4], g[5], g[6], g[7]};
+            geo[3 * it + 2] = float4{g[8], g[9], g[10], 0.f};
+        }
+        if ((rc = upload(c, (float **)&e.st_w, to32(L.st_w))) || (rc = upload(c, (float4 **)&e.it_f, geo)) ||
             (rc = upload(c, (float **)&e.b_dmag, to32(bdmag))) || (rc = upload(c, (float **)&e.corr, to32(L.corr))) ||
             (rc = upload(c, (DevPatch **)&e.patches, dp32)))
             return fail(rc);

@@ -215,14 +215,16 @@ inline size_t flux_lds_bytes(int max_items, int block_cells, bool stage, int C =
 template <typename S, typename LD>
 __device__ __forceinline__ void item_flux(const DevMesh<S> &m, const DevParams &prm, int64_t e, const LD &ld,
                                           double v3g, S F[4]) {
-    const int64_t E = m.E;
-    const int o = m.it_own[e];
-    const int nb = m.it_nbr[e];
+    const int4 ix = m.it_x[e];
+    const int o = ix.x;
+    const int nb = ix.y;
+    const unsigned side = (unsigned)ix.z;
     S geo[kGeoPerItem];  // n(3), A, dt, r_owner(3), r_neighbour(3)
     if constexpr (std::is_same<S, double>::value) {
-#pragma unroll
-        for (int q = 0; q < 5; q++) geo[q] = m.it_geo[(size_t)q * E + e];
-        const double fx = m.it_fc[e], fy = m.it_fc[(size_t)E + e], fz = m.it_fc[2 * (size_t)E + e];
+        const double4 g0 = m.it_d[2 * e], g1 = m.it_d[2 * e + 1];
+        geo[0] = g0.x; geo[1] = g0.y; geo[2] = g0.z; geo[3] = g0.w;
+        geo[4] = g1.x;
+        const double fx = g1.y, fy = g1.z, fz = g1.w;
         const double4 co = ld.centroid(o);
         geo[5] = co.x - fx;
         geo[6] = co.y - fy;
@@ -238,8 +240,10 @@ __device__ __forceinline__ void item_flux(const DevMesh<S> &m, const DevParams &
             geo[10] = m.g0z - fz;
         }
     } else {
-#pragma unroll
-        for (int q = 0; q < kGeoPerItem; q++) geo[q] = m.it_geo[(size_t)q * E + e];
+        const float4 g0 = m.it_f[3 * e], g1 = m.it_f[3 * e + 1], g2 = m.it_f[3 * e + 2];
+        geo[0] = g0.x; geo[1] = g0.y; geo[2] = g0.z; geo[3] = g0.w;
+        geo[4] = g1.x; geo[5] = g1.y; geo[6] = g1.z; geo[7] = g1.w;
+        geo[8] = g2.x; geo[9] = g2.y; geo[10] = g2.z;
     }
     const S n[3] = {geo[0], geo[1], geo[2]};
     const S A = geo[3], dt = geo[4];
@@ -294,7 +298,7 @@ __device__ __forceinline__ void item_flux(const DevMesh<S> &m, const DevParams &
         for (int q = 0; q < 4; q++) pI[q] = pN[q] = p0[q] = fv[q];
     }
     const S tau = (S)v3g / dt;
-    lattice_flux<std::is_same<S, float>::value, S>(n, A, dt, tau, m.it_side[e], p0, pI, go, pN, gN, (S)prm.invgam,
+    lattice_flux<std::is_same<S, float>::value, S>(n, A, dt, tau, side, p0, pI, go, pN, gN, (S)prm.invgam,
                                                    (S)prm.rho0, F);
 }
 

@@ -154,53 +154,56 @@ def run_batch(args):
 
 
 def run_to_tolerance(args):
-    """Wall time to max(u, v, w) residual 1e-6 from the uniform start, plain
-    FP64 and FP32 storage, on the chosen workload; velocity rel-L2 of FP32
-    against FP64 at convergence.  One JSON line."""
+    """Wall time to max(u, v, w) residual 1e-6 on the chosen workload: plain
+    FP64 and FP32 storage from the uniform start, and FP32 from the learned
+    warm start (trained parameters shipped in plbfs/data, or --params) with
+    its probe-based fallback; velocity rel-L2 of each FP32 result against
+    FP64 at convergence.  One JSON line."""
+    from plbfs import warmstart as ws
     pm, case, desc, ibm = workload(args.workload)
     if args.no_ibm:
         ibm = None
     hm = HostMesh.from_polymesh(pm, case.visc, case.gamma, case.fneq_min)
     cells = hm.geometry()
-    st = case.initial_state(cells["cx"], cells["cy"], cells["cz"])
-    out = {}
-    states = {}
-    for prec in ("fp64", "fp32"):
+    N = pm.n_cells
+    st = case.initial_state(cells["cx"][:N], cells["cy"][:N], cells["cz"][:N])
+    draw = None
+    ibm_kw = None
+    if ibm:
+        draw = ws.Draw(case.reynolds_no, 2.0 * ibm["radius"], tuple(ibm["centre"]), nodes=ibm["n_nodes"])
+        ibm_kw = ws.ibm_args(draw, case, stiffness=ibm["stiffness"], h=ibm["h"])
+        ibm_kw["kernel"] = ibm["kernel"]
+    out, states = {}, {}
+    params = ws.load_params(args.params or None) if draw is not None else None
+    legs = [("fp64", 0, False), ("fp32", 1, False)] + ([("fp32_warm", 1, True)] if params is not None else [])
+    for name, prec, warm in legs:
         case.tolerance = args.tol
         case.check_interval = 100
-        case.precision = 1 if prec == "fp32" else 0
-        s = GpuSolver(hm, case.params, case.patches_for(pm.patch_names))
-        s.set_state(*st)
-        if ibm:
-            X, dA = objects.fibonacci_sphere(ibm["n_nodes"], ibm["centre"], ibm["radius"])
-            s.set_ibm(X, X, stiffness=ibm["stiffness"], node_area=dA, dt=case.time_marching_step, h=ibm["h"],
-                      kernel=ibm["kernel"])
-        # chunks of 5000 steps, so that progress is printed while a long solve runs
-        t = time.perf_counter()
-        hist, dev_ms, steps = [], 0.0, 0
-        while steps < args.max_steps:
-            r = s.run(min(5000, args.max_steps - steps))
-            hist.append(r["hist"])
-            dev_ms += r["ms"]
-            steps += r["steps"]
-            print("  %s: %d steps, residual %.3e" % (prec, steps, max(r["residual"][1:])), file=sys.stderr,
-                  flush=True)
-            if r["converged"] or r["rc"] != 0:
-                break
-        wall = time.perf_counter() - t
-        r = dict(r, steps=steps, ms=dev_ms)
-        h = np.concatenate(hist)
-        marks = {str(k): float(max(h[k - 1, 1:])) for k in (1, 100, 1000, 5000, 10000, 20000, 40000, 80000)
-                 if k <= len(h)}
-        out[prec] = dict(steps=r["steps"], converged=r["converged"], rc=r["rc"], seconds=wall,
-                         device_ms=r["ms"], residual=max(r["residual"][1:]), history=marks)
-        print("%s: %d steps, converged %s, %.1f s" % (prec, r["steps"], r["converged"], wall), file=sys.stderr,
-              flush=True)
-        states[prec] = s.state()
-        s.close()
+        case.precision = prec
+        if warm:
+            net = ws.network_for(hm, N, params)
+            r, state, info = ws.warm_solve(pm, hm, case, ibm_kw, net, draw, args.max_steps, probe=args.probe,
+                                           label=name)
+            net.close()
+        else:
+            r, state = ws.solve_to_tolerance(pm, hm, case, st, ibm_kw, args.max_steps, label=name)
+            info = None
+        out[name] = dict(steps=r["steps"], converged=r["converged"], rc=r["rc"], seconds=r["seconds"],
+                         device_ms=r["ms"], residual=r["residual"])
+        if info:
+            out[name]["start"] = info
+        print("%s: %d steps, converged %s, %.1f s" % (name, r["steps"], r["converged"], r["seconds"]),
+              file=sys.stderr, flush=True)
+        states[name] = state
     u64 = np.stack(states["fp64"][1:], 1)
-    u32 = np.stack(states["fp32"][1:], 1)
-    out["fp32_velocity_rel_l2_vs_fp64"] = float(np.linalg.norm(u32 - u64) / np.linalg.norm(u64))
+    for name in states:
+        if name != "fp64":
+            u = np.stack(states[name][1:], 1)
+            out[name]["velocity_rel_l2_vs_fp64"] = float(np.linalg.norm(u - u64) / np.linalg.norm(u64))
+    if "fp32_warm" in out and out["fp32_warm"]["converged"] and out["fp64"]["converged"]:
+        out["speedup_fp64_rest_over_fp32_warm"] = out["fp64"]["seconds"] / out["fp32_warm"]["seconds"]
+    if out["fp32"]["converged"] and out["fp64"]["converged"]:
+        out["speedup_fp64_rest_over_fp32_rest"] = out["fp64"]["seconds"] / out["fp32"]["seconds"]
     out["workload"] = desc
     out["tolerance"] = args.tol
     print(json.dumps(out))
@@ -221,6 +224,8 @@ def main():
                     help="measure wall time to the residual tolerance (FP64 and FP32) instead of throughput")
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--max-steps", type=int, default=60000)
+    ap.add_argument("--params", default="", help="trained warm-start parameters (.npy); default plbfs/data")
+    ap.add_argument("--probe", type=int, default=200, help="steps of the warm-start fallback probe")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a device copy of 1 GiB first (PMC FETCH/WRITE calibration)")
     args = ap.parse_args()

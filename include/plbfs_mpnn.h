@@ -42,8 +42,9 @@ int plbfs_mpnn_loss_grad(plbfs_mpnn *net, const float *x, const float *target, c
 /* One Adam step on the gradient of the last plbfs_mpnn_loss_grad. */
 int plbfs_mpnn_adam_step(plbfs_mpnn *net, float lr, float beta1, float beta2, float eps);
 /* epochs passes over n_samples (x, target) pairs held on the device, one Adam
- * step per sample in a seeded shuffled order; loss_hist[epochs] gets the mean
- * loss of each epoch. */
+ * step per sample in a seeded shuffled order; the step size follows a cosine
+ * from lr in the first epoch down to lr/20 in the last.  loss_hist[epochs] gets
+ * the mean loss of each epoch.  The sample copies are freed on return. */
 int plbfs_mpnn_train(plbfs_mpnn *net, int32_t n_samples, const float *X, const float *T, const float *mask,
                      int32_t epochs, float lr, uint64_t seed, double *loss_hist);
 

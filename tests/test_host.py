@@ -267,3 +267,33 @@ def test_spring_network_object_nodes(tmp_path):
     rc = L.plbfs_case_object_nodes(h, 0, nat.dptr(x), 4, C.byref(n), C.byref(st), C.byref(area), C.byref(k))
     assert rc == -1
     L.plbfs_case_free(h)
+
+
+def test_warm_start_union_graph_and_draws():
+    from plbfs import mpnn
+    from plbfs import warmstart as ws
+    parts = []
+    for tets in (False, True):
+        pm = meshgen.box_mesh(4, 3, 2, tets=tets, jitter=0.1, jitter_seed=1, permute_seed=2)
+        hm = HostMesh.from_polymesh(pm, 0.01)
+        parts.append((hm.geometry(), pm.n_cells))
+    g = ws.UnionGraph(parts)
+    single = [mpnn.graph_from_geometry(geo, n) for geo, n in parts]
+    assert g.N == parts[0][1] + parts[1][1]
+    assert len(g.src) == sum(len(s[0]) for s in single)
+    e0 = len(single[0][0])
+    assert np.array_equal(g.src[:e0], single[0][0]) and np.array_equal(g.src[e0:], single[1][0] + parts[0][1])
+    assert np.array_equal(g.dst[e0:], single[1][1] + parts[0][1])
+    assert g.src[:e0].max() < parts[0][1] <= g.src[e0:].min()
+    assert np.array_equal(g.ef[e0:], single[1][2])
+    # every directed edge has its reverse with negated features
+    key = {(s, d): k for k, (s, d) in enumerate(zip(g.src, g.dst))}
+    for k, (s, d) in enumerate(zip(g.src, g.dst)):
+        assert np.array_equal(g.ef[key[(d, s)]], -g.ef[k])
+    # sphere sizes scale with the channel height; centres in the upstream third
+    for L in ((64.0, 32.0, 32.0), (176.0, 88.0, 88.0)):
+        dr = ws.sample_cases(200, L, seed=3, d_frac=(0.14, 0.44))
+        D = np.array([d.diameter for d in dr])
+        assert D.min() >= 0.14 * L[1] and D.max() <= 0.44 * L[1]
+        assert all(0.5 * d.diameter < d.centre[0] <= L[0] / 3 for d in dr)
+        assert all(abs(d.centre[1] - 0.5 * L[1]) <= L[1] / 16 for d in dr)
