@@ -39,6 +39,7 @@ struct BatchArgs {
     int stage;              // 1: stage the tile's own (cell, case) records in LDS
     const double *visc;     // [B] nu
     const double *v3g;      // [B] 3 nu / gamma
+    const double *dtcap;    // [B] streaming dt cap 3 nu / (gamma fneq_min), or infinity
     const int32_t *done;    // [B] frozen cases
 };
 
@@ -100,7 +101,7 @@ __global__ __launch_bounds__(kBlock) void kb_flux(DevMesh<S> m, DevParams prm, B
         if (b >= B || a.done[b]) continue;
         S F[4];
         const SideLoader<S, RecB> ld{st_in, grad, m.cgeo, RecB{B, b}, sRec, sCg, c0, nst, C, c};
-        item_flux<S>(m, prm, e0 + el, ld, a.v3g[b], F);
+        item_flux<S>(m, prm, e0 + el, ld, a.v3g[b], a.dtcap[b], F);
 #pragma unroll
         for (int q = 0; q < 4; q++) sF[q * stride + el * C + c] = F[q];
     }
@@ -266,7 +267,7 @@ struct plbfs_batch {
     void *stA = nullptr, *stB = nullptr, *grad = nullptr, *t1 = nullptr;
     double *t0 = nullptr, *dtl = nullptr, *fbody = nullptr, *partials = nullptr, *scratch = nullptr;
     size_t partials_cap = 0;
-    double *d_visc = nullptr, *d_v3g = nullptr;
+    double *d_visc = nullptr, *d_v3g = nullptr, *d_dtcap = nullptr;
     int32_t *done = nullptr, *status = nullptr, *steps = nullptr, *conv = nullptr;
     double *res = nullptr;
     Ctrl *ctrl = nullptr;
@@ -357,7 +358,9 @@ int retile(plbfs_batch *b, int block_cells, int C, int stage) {
     return PLBFS_OK;
 }
 
-BatchArgs args_of(plbfs_batch *b) { return BatchArgs{b->B, b->C, b->stage, b->d_visc, b->d_v3g, b->done}; }
+BatchArgs args_of(plbfs_batch *b) {
+    return BatchArgs{b->B, b->C, b->stage, b->d_visc, b->d_v3g, b->d_dtcap, b->done};
+}
 
 int ibm_upload(plbfs_batch *b) {
     b->ibm_dirty = false;
@@ -575,8 +578,6 @@ extern "C" int plbfs_batch_create(const plbfs_mesh_view *mesh, const plbfs_param
     *out = nullptr;
     if (params->time_stepping != PLBFS_DT_LOCAL && params->time_stepping != PLBFS_DT_CONSTANT)
         return plbfs_fail(PLBFS_ERR_INPUT, "batched cases support local and constant time stepping");
-    if (params->fneq_min != 0.0)
-        return plbfs_fail(PLBFS_ERR_INPUT, "batched cases need fneq_min = 0 (the face dt cap would depend on nu)");
     for (int c = 0; c < n_cases; c++)
         if (!(case_visc[c] > 0)) return plbfs_fail(PLBFS_ERR_INPUT, "case %d: viscosity must be positive", c);
     auto *b = new plbfs_batch();
@@ -593,6 +594,11 @@ extern "C" int plbfs_batch_create(const plbfs_mesh_view *mesh, const plbfs_param
     b->fn.assign(mesh->face_normal, mesh->face_normal + 3 * (size_t)F);
     b->fa.assign(mesh->face_area, mesh->face_area + F);
     b->fdt.assign(mesh->face_dt, mesh->face_dt + F);
+    if (params->fneq_min != 0.0) {
+        // the cap 3 nu / (gamma fneq_min) differs per case: keep the geometric
+        // streaming dt here (the host mesh formula) and cap it per case on the device
+        for (int f = 0; f < F; f++) b->fdt[f] = 0.5 * std::sqrt(b->fa[f]);
+    }
     if (bface_patch) b->bpatch.assign(bface_patch, bface_patch + (F - NI));
     b->params = *params;
     b->patches.assign(patches, patches + n_patches);
@@ -609,7 +615,11 @@ extern "C" int plbfs_batch_create(const plbfs_mesh_view *mesh, const plbfs_param
     const size_t NR = (size_t)N * n_cases;
     const size_t sb = b->f32 ? 4 : 8;
     std::vector<double> v3g(n_cases);
-    for (int c = 0; c < n_cases; c++) v3g[c] = case_visc[c] * 3 / params->gamma;
+    std::vector<double> dtcap(n_cases, INFINITY);
+    for (int c = 0; c < n_cases; c++) {
+        v3g[c] = case_visc[c] * 3 / params->gamma;
+        if (params->fneq_min != 0.0) dtcap[c] = case_visc[c] * 3 / params->gamma / params->fneq_min;
+    }
     if ((rc = balloc(b, &b->stA, NR * 4 * sb)) || (rc = balloc(b, &b->stB, NR * 4 * sb)) ||
         (rc = balloc(b, &b->grad, NR * 12 * sb)) || (rc = balloc(b, &b->t1, NR * 4 * sb)) ||
         (rc = balloc(b, (void **)&b->t0, NR * 4 * sizeof(double))) ||
@@ -617,6 +627,7 @@ extern "C" int plbfs_batch_create(const plbfs_mesh_view *mesh, const plbfs_param
         (rc = balloc(b, (void **)&b->fbody, NR * 3 * sizeof(double))) ||
         (rc = balloc(b, (void **)&b->scratch, (size_t)N * 4 * sizeof(double))) ||
         (rc = bupload(b, &b->d_visc, b->visc)) || (rc = bupload(b, &b->d_v3g, v3g)) ||
+        (rc = bupload(b, &b->d_dtcap, dtcap)) ||
         (rc = balloc(b, (void **)&b->done, n_cases * sizeof(int32_t))) ||
         (rc = balloc(b, (void **)&b->status, n_cases * sizeof(int32_t))) ||
         (rc = balloc(b, (void **)&b->steps, n_cases * sizeof(int32_t))) ||

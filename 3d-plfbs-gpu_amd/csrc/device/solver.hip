@@ -123,7 +123,7 @@ __global__ __launch_bounds__(kBlock) void k_flux_update(DevMesh<S> m, DevParams 
     const SideLoader<S, Rec1> ld = block_loader<S>(m, st_in, grad, lds, blk);
     for (int e = e0 + threadIdx.x; e < e1; e += kBlock) {
         S F[4];
-        item_flux<S>(m, prm, e, ld, prm.visc3_invgam, F);
+        item_flux<S>(m, prm, e, ld, prm.visc3_invgam, INFINITY, F);
         const int sl = e - e0;
 #pragma unroll
         for (int q = 0; q < 4; q++) sF[q * nslot + sl] = F[q];
@@ -279,7 +279,7 @@ __global__ __launch_bounds__(kBlock) void k_residual_only(DevMesh<S> m, DevParam
     const SideLoader<S, Rec1> ld = block_loader<S>(m, st_in, grad, lds, blk);
     for (int e = e0 + threadIdx.x; e < e1; e += kBlock) {
         S F[4];
-        item_flux<S>(m, prm, e, ld, prm.visc3_invgam, F);
+        item_flux<S>(m, prm, e, ld, prm.visc3_invgam, INFINITY, F);
 #pragma unroll
         for (int q = 0; q < 4; q++) sF[q * nslot + (e - e0)] = F[q];
     }

@@ -211,10 +211,13 @@ inline size_t flux_lds_bytes(int max_items, int block_cells, bool stage, int C =
 }
 
 // Flux of face work item e (src/Solver.cpp:286-347 with populate_lattice_macros
-// :840-1274); v3g = 3 nu / gamma of the case.
+// :840-1274); v3g = 3 nu / gamma of the case.  dtcap caps the item's streaming
+// dt: a batch stores the geometric 0.5 sqrt(A) and caps it per case with
+// 3 nu / (gamma fneq_min), the min the host applies for one case
+// (src/unstructured_mesh.cpp:979-983); a single case passes infinity.
 template <typename S, typename LD>
 __device__ __forceinline__ void item_flux(const DevMesh<S> &m, const DevParams &prm, int64_t e, const LD &ld,
-                                          double v3g, S F[4]) {
+                                          double v3g, double dtcap, S F[4]) {
     const int4 ix = m.it_x[e];
     const int o = ix.x;
     const int nb = ix.y;
@@ -246,7 +249,7 @@ __device__ __forceinline__ void item_flux(const DevMesh<S> &m, const DevParams &
         geo[8] = g2.x; geo[9] = g2.y; geo[10] = g2.z;
     }
     const S n[3] = {geo[0], geo[1], geo[2]};
-    const S A = geo[3], dt = geo[4];
+    const S A = geo[3], dt = fmin(geo[4], (S)dtcap);
     S po[4], go[12];
     ld.side(o, po, go);
     S pI[4], pN[4], p0[4], gN[12];

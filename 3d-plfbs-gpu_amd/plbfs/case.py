@@ -129,18 +129,22 @@ CHANNEL_PATCHES = {"x0": "inlet", "x1": "outlet", "y0": "farfield", "y1": "farfi
                    "z1": "farfield"}
 
 
-def channel_case(re=40.0, length_scale=12.0, mach=0.05, sides="farfield"):
+def channel_case(re=40.0, length_scale=12.0, mach=0.05, sides="farfield", fneq_min=0.0):
     """Channel: uniform inflow (velocity Dirichlet, density Neumann), outlet with
     zero-gradient velocity and fixed density.  Lateral sides: ``farfield`` holds
     the free-stream velocity (Dirichlet, density Neumann); ``symmetry`` uses the
     reference's only slip type symmetry-x (mirrors v); ``neumann`` is zero
     gradient.  The reference's Neumann ghost gradient (grad * (1 - n), without
     abs) makes zero-gradient sides unstable on these channels, so far-field is
-    the default.  ``length_scale`` is the Reynolds length (sphere diameter)."""
+    the default.  ``length_scale`` is the Reynolds length (sphere diameter).
+    ``fneq_min`` > 0 caps every face's streaming dt at 3 nu / (gamma fneq_min),
+    which keeps the non-equilibrium weight tau >= fneq_min
+    (src/unstructured_mesh.cpp:979-983); flows past an immersed sphere at cell
+    Reynolds numbers above ~5 diverge without it."""
     side = {"farfield": dict(rho=1.0, u=1.0, v=0.0, w=0.0, vel_type="dirichlet", rho_type="neumann"),
             "symmetry": dict(rho=1.0, u=0.0, v=0.0, w=0.0, vel_type="symmetry-x", rho_type="symmetry-x"),
             "neumann": dict(rho=1.0, u=0.0, v=0.0, w=0.0, vel_type="neumann", rho_type="neumann")}[sides]
-    return Case(mach_no=mach, reynolds_no=re, ref_length=length_scale,
+    return Case(mach_no=mach, reynolds_no=re, ref_length=length_scale, fneq_min=fneq_min,
                 vel_origin_magnitude=(1.0, 0.0, 0.0),
                 bcs={"inlet": dict(rho=1.0, u=1.0, v=0.0, w=0.0, vel_type="dirichlet", rho_type="neumann"),
                      "outlet": dict(rho=1.0, u=0.0, v=0.0, w=0.0, vel_type="neumann", rho_type="dirichlet"),

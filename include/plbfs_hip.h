@@ -207,8 +207,11 @@ int plbfs_ibm_body_force(plbfs_ctx *ctx, double *force3);
  * returns once every case is frozen or max_steps have passed.  The reference
  * has no batched solver: each reference run is one call of
  * gpu_solver::General_Purpose_Solver_mk_i (src/Solver_gpu.h:28-31), and a batch
- * replaces B such calls on one mesh.  Time stepping local or constant;
- * fneq_min must be 0 (the face dt then does not depend on the viscosity). */
+ * replaces B such calls on one mesh.  Time stepping local or constant.  With
+ * fneq_min > 0 the face streaming dt is min(0.5 sqrt(A), 3 nu_b / (gamma
+ * fneq_min)) per case b, the mesh builder's formula for that case's viscosity
+ * (the mesh view's face_dt is then not used); with fneq_min = 0 it is the
+ * view's face_dt. */
 typedef struct plbfs_batch plbfs_batch;
 
 int plbfs_batch_create(const plbfs_mesh_view *mesh, const plbfs_params *params, const plbfs_patch *patches,

@@ -19,10 +19,10 @@ pytestmark = pytest.mark.gpu
 RES = [30.0, 60.0, 120.0]
 
 
-def cases_for(res, precision=0, tol=1e-30, check=10):
+def cases_for(res, precision=0, tol=1e-30, check=10, fneq_min=0.0):
     out = []
     for re in res:
-        c = pc.channel_case(re=re, length_scale=4.0)
+        c = pc.channel_case(re=re, length_scale=4.0, fneq_min=fneq_min)
         c.precision = precision
         c.tolerance = tol
         c.check_interval = check
@@ -46,10 +46,10 @@ def single_run(pm, case, st, steps, ibm=None):
     return r, s.state()
 
 
-@pytest.mark.parametrize("precision", [0, 1])
-def test_batch_cases_equal_single_runs(gpu_available, precision):
+@pytest.mark.parametrize("precision,fneq_min", [(0, 0.0), (1, 0.0), (0, 0.55), (1, 0.55)])
+def test_batch_cases_equal_single_runs(gpu_available, precision, fneq_min):
     pm = channel_mesh(10, 5, 5)
-    cases = cases_for(RES, precision)
+    cases = cases_for(RES, precision, fneq_min=fneq_min)
     o = oracle_for(pm, cases[0])
     st = perturbed_state(cases[0], o, amp=0.01, seed=2)
     hm, bt = make_batch(pm, cases)
@@ -64,9 +64,12 @@ def test_batch_cases_equal_single_runs(gpu_available, precision):
         assert np.allclose(res[b]["residual"], r1["residual"], rtol=1e-12, atol=0)
 
 
-def test_batch_fp64_matches_oracle_per_case(gpu_available):
+@pytest.mark.parametrize("fneq_min", [0.0, 0.55])
+def test_batch_fp64_matches_oracle_per_case(gpu_available, fneq_min):
+    # with fneq_min > 0 each case's face streaming dt carries its own cap
+    # 3 nu / (gamma fneq_min), which the batch applies on the device
     pm = channel_mesh(10, 5, 5)
-    cases = cases_for(RES)
+    cases = cases_for(RES, fneq_min=fneq_min)
     o0 = oracle_for(pm, cases[0])
     st = perturbed_state(cases[0], o0, amp=0.01, seed=6)
     hm, bt = make_batch(pm, cases)

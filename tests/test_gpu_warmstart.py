@@ -3,8 +3,9 @@
 A small network is trained on the GPU (plbfs_mpnn_train) on converged fields
 that the batched solver produces for seeded (Re, sphere) draws on a small
 jittered channel.  For a held-out draw, the plain FP32 solver started from the
-network's prediction must reach the same residual tolerance in fewer steps
-than from the uniform start, and land on the same steady state.  An untrained
+network's prediction must reach the residual tolerance (1e-5 here, to keep the
+test short) in fewer steps than from the uniform start, and land on the same
+steady state.  An untrained
 network's prediction must lose the probe, so that the solve falls back to the
 uniform start and gives exactly the plain solve's fields.
 """
@@ -19,6 +20,7 @@ from plbfs.solver import HostMesh
 pytestmark = pytest.mark.gpu
 
 L = (32.0, 16.0, 16.0)
+TOL = 1e-5
 
 
 @pytest.fixture(scope="module")
@@ -37,7 +39,7 @@ def draws(n, seed):
 def trained(channel, gpu_available):
     pm, hm = channel
     train = draws(24, seed=31)
-    fields, res = ws.generate(pm, hm, train, 30000)
+    fields, res = ws.generate(pm, hm, train, 30000, tol=TOL)
     keep = [k for k, r in enumerate(res) if ws.usable(r)]
     assert len(keep) >= 20, [(r["status"], r["residual"]) for r in res]
     geo = hm.geometry()
@@ -60,12 +62,13 @@ def held_out():
 def test_warm_start_cuts_steps(channel, trained):
     pm, hm = channel
     d = held_out()
-    case = ws.case_for(d, precision=1)
+    case = ws.case_for(d, precision=1, tol=TOL)
     ibm = ws.ibm_args(d, case)
     geo = hm.geometry()
-    rest, s_rest = ws.solve_to_tolerance(pm, hm, case, ws.uniform_state(case, geo, pm.n_cells), ibm, 40000)
+    hc = ws.mesh_for(pm, case)
+    rest, s_rest = ws.solve_to_tolerance(pm, hc, case, ws.uniform_state(case, geo, pm.n_cells), ibm, 80000)
     net = ws.network_for(hm, pm.n_cells, trained)
-    warm, s_warm, info = ws.warm_solve(pm, hm, case, ibm, net, d, 40000, probe=100)
+    warm, s_warm, info = ws.warm_solve(pm, hc, case, ibm, net, d, 80000, probe=100)
     net.close()
     assert rest["converged"] and warm["converged"], (rest, warm)
     assert info["used"] == "prediction", info
@@ -79,12 +82,13 @@ def test_warm_start_cuts_steps(channel, trained):
 def test_untrained_network_falls_back(channel, gpu_available):
     pm, hm = channel
     d = held_out()
-    case = ws.case_for(d, precision=1)
+    case = ws.case_for(d, precision=1, tol=TOL)
     ibm = ws.ibm_args(d, case)
     geo = hm.geometry()
-    rest, s_rest = ws.solve_to_tolerance(pm, hm, case, ws.uniform_state(case, geo, pm.n_cells), ibm, 40000)
+    hc = ws.mesh_for(pm, case)
+    rest, s_rest = ws.solve_to_tolerance(pm, hc, case, ws.uniform_state(case, geo, pm.n_cells), ibm, 80000)
     net = ws.network_for(hm, pm.n_cells, mpnn.init_params(11) * 4.0)
-    warm, s_warm, info = ws.warm_solve(pm, hm, case, ibm, net, d, 40000, probe=100)
+    warm, s_warm, info = ws.warm_solve(pm, hc, case, ibm, net, d, 80000, probe=100)
     net.close()
     assert info["used"] == "uniform", info
     assert warm["steps"] == rest["steps"]
