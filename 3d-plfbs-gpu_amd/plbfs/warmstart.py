@@ -39,7 +39,7 @@ from .solver import GpuBatch, GpuSolver, HostMesh
 
 # Spring stiffness of the tether between each Lagrangian node and its
 # reference position (force = -k dA (X - X_ref)).
-STIFFNESS = 2.0
+STIFFNESS = 5.0
 # Floor of the non-equilibrium weight tau through the face streaming dt cap
 # (src/unstructured_mesh.cpp:979-983, whose comment gives 0.55 as the
 # conservative stability bound).  Without it these sphere flows diverge once
@@ -172,9 +172,9 @@ def generate(pm, hm, draws, max_steps, tiling=None, precision=1, lsq_exact_det=0
     return fields, res
 
 
-def usable(result, tol=1e-4):
+def usable(result, tol=1e-3):
     """A field is a training target when its solve ended cleanly at a residual
-    below tol (converged to 1e-6, or close to it when the step cap came first)."""
+    below tol (converged, or nearly steady when the step cap came first)."""
     return result["status"] == 0 and max(result["residual"][1:]) < tol
 
 
@@ -202,6 +202,22 @@ class UnionGraph:
         self.dst = np.concatenate(dst).astype(np.int32)
         self.ef = np.concatenate(ef).astype(np.float32)
         self.N = int(self.off[-1])
+
+
+def train_epochs(net, X, T, mask, epochs, lr, seed):
+    """One plbfs_mpnn_train call per epoch (so that progress is printed), with a
+    cosine step size from lr down to lr/20 over the epochs; the Adam moments
+    persist in the network between calls."""
+    hist = []
+    for ep in range(epochs):
+        ph = ep / (epochs - 1) if epochs > 1 else 0.0
+        lr_ep = lr / 20 + 0.5 * (lr - lr / 20) * (1 + math.cos(math.pi * ph))
+        t = time.perf_counter()
+        h = net.train(X, T, mask, epochs=1, lr=lr_ep, seed=seed + ep)
+        hist.append(float(h[0]))
+        print("  epoch %d/%d: loss %.4g (lr %.2e, %.1f s)" % (ep + 1, epochs, hist[-1], lr_ep,
+                                                            time.perf_counter() - t), file=sys.stderr, flush=True)
+    return np.array(hist)
 
 
 def predict(net, geo, n, draw):
@@ -323,7 +339,7 @@ def train_main(args):
         draws = sample_cases(count, L, seed=args.seed + len(parts), d_frac=(args.d_lo, args.d_hi))
         fields, keep = [], []
         for k in range(0, count, args.batch):
-            f, res = generate(pm, hm, draws[k:k + args.batch], args.max_steps, lsq_exact_det=lsq_exact(name),
+            f, res = generate(pm, hm, draws[k:k + args.batch], args.gen_steps, lsq_exact_det=lsq_exact(name),
                               stiffness=args.stiffness)
             for d, fld, r in zip(draws[k:k + args.batch], f, res):
                 if usable(r):
@@ -347,7 +363,7 @@ def train_main(args):
     net = mpnn.Mpnn(g.N, g.src, g.dst, g.ef, train=True)
     net.set_params(mpnn.init_params(args.seed))
     t1 = time.perf_counter()
-    hist = net.train(X, T, mpnn.output_mask(g.N), epochs=args.epochs, lr=args.lr, seed=args.seed)
+    hist = train_epochs(net, X, T, mpnn.output_mask(g.N), args.epochs, args.lr, args.seed)
     t_train = time.perf_counter() - t1
     print("trained %d epochs on %d samples in %.1f s, loss %.4g -> %.4g" % (
         args.epochs, S, t_train, hist[0], hist[-1]), file=sys.stderr, flush=True)
@@ -407,7 +423,8 @@ def main(argv=None):
     ap.add_argument("--eval", default="", help="mesh of the held-out comparison (cfg4, tet32, chan96, tet88)")
     ap.add_argument("--test", type=int, default=8)
     ap.add_argument("--batch", type=int, default=128)
-    ap.add_argument("--max-steps", type=int, default=40000)
+    ap.add_argument("--max-steps", type=int, default=60000, help="step cap of the held-out solves")
+    ap.add_argument("--gen-steps", type=int, default=20000, help="step cap of the training-field batches")
     ap.add_argument("--epochs", type=int, default=30)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--seed", type=int, default=7)

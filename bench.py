@@ -42,7 +42,7 @@ def workload(name):
         case = pc.channel_case(re=100.0, length_scale=0.15 * n, fneq_min=0.55)
         case.lsq_exact_det = 1  # the reference's LSQ determinant diverges on jittered tets
         desc = "88^3x6 Kuhn tets (4,088,832 cells) 2:1:1 channel, jitter 10%, permuted, Re=100"
-        ibm = dict(n_nodes=2000, centre=(0.5 * n, 0.5 * n, 0.5 * n), radius=0.075 * n, stiffness=2.0, h=1.0,
+        ibm = dict(n_nodes=2000, centre=(0.5 * n, 0.5 * n, 0.5 * n), radius=0.075 * n, stiffness=5.0, h=1.0,
                    kernel=1)
     elif name == "hex32":
         pm = meshgen.box_mesh(32, 32, 32, patch_types=pc.CAVITY_PATCHES)
@@ -54,7 +54,7 @@ def workload(name):
                               permute_seed=3, patch_types=pc.CHANNEL_PATCHES)
         case = pc.channel_case(re=40.0, length_scale=12.0, fneq_min=0.55)
         desc = "channel 96x48x48 hex (221,184 cells), jitter 15%, permuted, Re=40"
-        ibm = dict(n_nodes=450, centre=(24.0, 24.0, 24.0), radius=6.0, stiffness=2.0, h=1.0, kernel=1)
+        ibm = dict(n_nodes=450, centre=(24.0, 24.0, 24.0), radius=6.0, stiffness=5.0, h=1.0, kernel=1)
     else:
         raise SystemExit("unknown workload %s" % name)
     return pm, case, desc, ibm
