@@ -1,7 +1,11 @@
 """Warm-start network front end (include/plbfs_mpnn.h) and its features.
 
-Node inputs (16 per cell, 5 used): centroid scaled to [-1, 1] per axis, signed
-distance to the immersed sphere over its diameter, normalised Reynolds number.
+Node inputs (16 per cell, 9 used): centroid scaled to [-1, 1] per axis, signed
+distance to the immersed sphere over its diameter, normalised Reynolds number,
+the offset from the sphere centre over its diameter (clipped to +-8, over 4),
+and the sphere diameter over the channel height (centred on 0.3, over 0.15).
+The offset tells the network which side of the sphere a cell is on (upstream
+or in the wake), which the distance alone does not.
 Edge inputs (16 per directed interior face, 6 used): the face's n A over h^2
 and the centroid offset x_src - x_dst over h, with h the mean cell size; the
 reverse edge flips both.  Outputs (16, 4 used): (rho - 1) / RHO_SCALE and
@@ -53,6 +57,8 @@ def node_inputs(geo, n_cells, centre, diameter, re):
     x[:, 0:3] = 2 * (cx - lo) / np.maximum(hi - lo, 1e-30) - 1
     x[:, 3] = (np.linalg.norm(cx - np.asarray(centre), axis=1) - 0.5 * diameter) / diameter
     x[:, 4] = (re - RE_MID) / RE_HALF
+    x[:, 5:8] = np.clip((cx - np.asarray(centre)) / diameter, -8.0, 8.0) / 4.0
+    x[:, 8] = (diameter / max(hi[1] - lo[1], 1e-30) - 0.3) / 0.15
     return x
 
 

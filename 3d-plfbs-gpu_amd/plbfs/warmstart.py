@@ -150,13 +150,15 @@ def generate(pm, hm, draws, max_steps, tiling=None, precision=1, lsq_exact_det=0
     for c in cases:
         c.lsq_exact_det = lsq_exact_det
     bt = GpuBatch(hm, cases[0].params, cases[0].patches_for(pm.patch_names), [c.visc for c in cases])
-    if tiling:
-        bt.set_tiling(*tiling)
     geo = hm.geometry()
     N = pm.n_cells
     bt.set_state(-1, *uniform_state(cases[0], geo, N))
     for b, (d, c) in enumerate(zip(draws, cases)):
         bt.set_ibm(b, **ibm_args(d, c, stiffness))
+    if tiling == "auto":
+        bt.autotune(3)
+    elif tiling:
+        bt.set_tiling(*tiling)
     done = 0
     while done < max_steps:
         n = min(2500, max_steps - done)
@@ -204,10 +206,11 @@ class UnionGraph:
         self.N = int(self.off[-1])
 
 
-def train_epochs(net, X, T, mask, epochs, lr, seed):
+def train_epochs(net, X, T, mask, epochs, lr, seed, save=""):
     """One plbfs_mpnn_train call per epoch (so that progress is printed), with a
     cosine step size from lr down to lr/20 over the epochs; the Adam moments
-    persist in the network between calls."""
+    persist in the network between calls.  With `save`, the parameters are
+    written there after every epoch."""
     hist = []
     for ep in range(epochs):
         ph = ep / (epochs - 1) if epochs > 1 else 0.0
@@ -215,6 +218,8 @@ def train_epochs(net, X, T, mask, epochs, lr, seed):
         t = time.perf_counter()
         h = net.train(X, T, mask, epochs=1, lr=lr_ep, seed=seed + ep)
         hist.append(float(h[0]))
+        if save:
+            np.save(save, net.params())
         print("  epoch %d/%d: loss %.4g (lr %.2e, %.1f s)" % (ep + 1, epochs, hist[-1], lr_ep,
                                                             time.perf_counter() - t), file=sys.stderr, flush=True)
     return np.array(hist)
@@ -339,8 +344,8 @@ def train_main(args):
         draws = sample_cases(count, L, seed=args.seed + len(parts), d_frac=(args.d_lo, args.d_hi))
         fields, keep = [], []
         for k in range(0, count, args.batch):
-            f, res = generate(pm, hm, draws[k:k + args.batch], args.gen_steps, lsq_exact_det=lsq_exact(name),
-                              stiffness=args.stiffness)
+            f, res = generate(pm, hm, draws[k:k + args.batch], args.gen_steps, tiling="auto",
+                              lsq_exact_det=lsq_exact(name), stiffness=args.stiffness)
             for d, fld, r in zip(draws[k:k + args.batch], f, res):
                 if usable(r):
                     fields.append(fld)
@@ -363,7 +368,7 @@ def train_main(args):
     net = mpnn.Mpnn(g.N, g.src, g.dst, g.ef, train=True)
     net.set_params(mpnn.init_params(args.seed))
     t1 = time.perf_counter()
-    hist = train_epochs(net, X, T, mpnn.output_mask(g.N), args.epochs, args.lr, args.seed)
+    hist = train_epochs(net, X, T, mpnn.output_mask(g.N), args.epochs, args.lr, args.seed, save=args.save)
     t_train = time.perf_counter() - t1
     print("trained %d epochs on %d samples in %.1f s, loss %.4g -> %.4g" % (
         args.epochs, S, t_train, hist[0], hist[-1]), file=sys.stderr, flush=True)
