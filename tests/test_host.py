@@ -297,3 +297,11 @@ def test_warm_start_union_graph_and_draws():
         assert D.min() >= 0.14 * L[1] and D.max() <= 0.44 * L[1]
         assert all(0.5 * d.diameter < d.centre[0] <= L[0] / 3 for d in dr)
         assert all(abs(d.centre[1] - 0.5 * L[1]) <= L[1] / 16 for d in dr)
+
+
+def test_shipped_warm_start_parameters():
+    from plbfs import warmstart as ws
+    p = ws.load_params()
+    assert p is not None, "plbfs/data/mpnn_channel.npy missing"
+    assert p.dtype == np.float32 and p.shape == (nat.lib().plbfs_mpnn_n_params(),)
+    assert np.all(np.isfinite(p)) and 0 < np.abs(p).max() < 100
