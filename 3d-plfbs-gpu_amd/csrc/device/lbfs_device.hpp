@@ -194,31 +194,38 @@ struct Moments {
     T s0, sx, sy, sz, pxx, pyy, pzz, pxy, pxz, pyz;
 };
 
-// Equilibrium of one lattice point folded into the running moments.  With
-// DEV (FP32 storage) the density is a deviation r = rho - rho0 and only
+// Equilibrium of one lattice point e at state (r, u, v, w).  With DEV (FP32
+// storage) the density is a deviation r = rho - rho0 and the value returned is
 // g = feq - w rho0 = w (r (1 + X) + rho0 X), X = 3 e.u + (4.5 (e.u)^2 - 1.5 u^2)/gamma,
-// is accumulated, so the O(1) part rho0 w never enters single precision.
+// so the O(1) part rho0 w never enters single precision.
 template <int EX, int EY, int EZ, bool DEV, typename T>
-__device__ __forceinline__ void add_pop(Moments<T> &M, T wk, T r, T u, T v, T w, T invgam, T rho0) {
+__device__ __forceinline__ T pop_eq(T wk, T r, T u, T v, T w, T c45, T c15, T rho0) {
     const T eu = (T)EX * u + (T)EY * v + (T)EZ * w;
     const T usq = u * u + v * v + w * w;
-    T f;
+    // X = 3 e.u + (4.5 (e.u)^2 - 1.5 u^2) / gamma, as e.u (3 + c45 e.u) - c15 u^2
+    // with c45 = 4.5 / gamma and c15 = 1.5 / gamma
+    const T X = eu * ((T)3 + c45 * eu) - c15 * usq;
     if constexpr (DEV) {
-        const T X = (T)3 * eu + ((T)4.5 * eu * eu - (T)1.5 * usq) * invgam;
-        f = wk * (r * ((T)1 + X) + rho0 * X);
+        return wk * (r + (r + rho0) * X);
     } else {
-        f = wk * r * ((T)1 + (T)3 * eu + ((T)4.5 * eu * eu - (T)1.5 * usq) * invgam);
+        const T wr = wk * r;
+        return wr + wr * X;
     }
-    M.s0 += f;
-    if (EX) M.sx += (T)EX * f;
-    if (EY) M.sy += (T)EY * f;
-    if (EZ) M.sz += (T)EZ * f;
-    if (EX) M.pxx += f;
-    if (EY) M.pyy += f;
-    if (EZ) M.pzz += f;
-    if (EX * EY) M.pxy += (T)(EX * EY) * f;
-    if (EX * EZ) M.pxz += (T)(EX * EZ) * f;
-    if (EY * EZ) M.pyz += (T)(EY * EZ) * f;
+}
+
+// e.g of one lattice direction from a gradient triple g (x, y, z) and its sum
+// s = gx + gy + gz: the four body diagonals share s.
+template <int EX, int EY, int EZ, typename T>
+__device__ __forceinline__ T dir_deriv(const T *g, T s) {
+    if constexpr (EX != 0 && EY != 0 && EZ != 0) {
+        // EX, EY, EZ in {+-1}: at most one of them is -1 in kLatticePairs' order
+        if constexpr (EX < 0) return s - (T)2 * g[0];
+        else if constexpr (EY < 0) return s - (T)2 * g[1];
+        else if constexpr (EZ < 0) return s - (T)2 * g[2];
+        else return s;
+    } else {
+        return (T)EX * g[0] + (T)EY * g[1] + (T)EZ * g[2];
+    }
 }
 
 // Lattice pair +e / -e (pair index P, order of kLatticePairs).  Point +e takes
@@ -227,24 +234,35 @@ __device__ __forceinline__ void add_pop(Moments<T> &M, T wk, T r, T u, T v, T w,
 // decisions per pair are precomputed on the host in FP64 (bits 2P and 2P+1 of
 // `side`): for faces containing a lattice direction, e.n is zero analytically
 // and its sign is rounding noise, so evaluating it in another precision would
-// pick the other side.
+// pick the other side.  The pair enters the moments through its sum
+// S = f(+e) + f(-e) (even moments) and difference D = f(+e) - f(-e) (odd ones).
 template <int P, int EX, int EY, int EZ, bool DEV, typename T>
 __device__ __forceinline__ void lattice_pair(Moments<T> &M, T wk, unsigned side, T dt, const T pI[4],
-                                             const T gO[12], const T pN[4], const T gN[12], T invgam, T rho0) {
+                                             const T gO[12], const T sO[4], const T pN[4], const T gN[12],
+                                             const T sN[4], T c45, T c15, T rho0) {
     const bool up = (side >> (2 * P)) & 1u;      // +e point from the neighbour side
     const bool dn = (side >> (2 * P + 1)) & 1u;  // -e point from the neighbour side
     T a[4], b[4];
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-        T po = (T)0, pn = (T)0;
-        if (EX) { po += (T)EX * gO[3 * q]; pn += (T)EX * gN[3 * q]; }
-        if (EY) { po += (T)EY * gO[3 * q + 1]; pn += (T)EY * gN[3 * q + 1]; }
-        if (EZ) { po += (T)EZ * gO[3 * q + 2]; pn += (T)EZ * gN[3 * q + 2]; }
+        const T po = dir_deriv<EX, EY, EZ, T>(gO + 3 * q, sO[q]);
+        const T pn = dir_deriv<EX, EY, EZ, T>(gN + 3 * q, sN[q]);
         a[q] = up ? (pN[q] - pn * dt) : (pI[q] - po * dt);
         b[q] = dn ? (pN[q] + pn * dt) : (pI[q] + po * dt);
     }
-    add_pop<EX, EY, EZ, DEV, T>(M, wk, a[0], a[1], a[2], a[3], invgam, rho0);
-    add_pop<-EX, -EY, -EZ, DEV, T>(M, wk, b[0], b[1], b[2], b[3], invgam, rho0);
+    const T fp = pop_eq<EX, EY, EZ, DEV, T>(wk, a[0], a[1], a[2], a[3], c45, c15, rho0);
+    const T fm = pop_eq<-EX, -EY, -EZ, DEV, T>(wk, b[0], b[1], b[2], b[3], c45, c15, rho0);
+    const T S = fp + fm, D = fp - fm;
+    M.s0 += S;
+    if (EX) M.sx += (T)EX * D;
+    if (EY) M.sy += (T)EY * D;
+    if (EZ) M.sz += (T)EZ * D;
+    if (EX) M.pxx += S;
+    if (EY) M.pyy += S;
+    if (EZ) M.pzz += S;
+    if (EX * EY) M.pxy += (T)(EX * EY) * S;
+    if (EX * EZ) M.pxz += (T)(EX * EZ) * S;
+    if (EY * EZ) M.pyz += (T)(EY * EZ) * S;
 }
 
 // Flux through a face of area A and unit normal n (owner -> neighbour).  With
@@ -256,14 +274,21 @@ __device__ __forceinline__ void lattice_flux(const T n[3], T A, T dt, T tau, uns
                                              T rho0, T F[4]) {
     Moments<T> M = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     const T w0 = (T)(2.0 / 9.0), w1 = (T)(1.0 / 9.0), w7 = (T)(1.0 / 72.0);
-    add_pop<0, 0, 0, DEV, T>(M, w0, p0[0], p0[1], p0[2], p0[3], invgam, rho0);
-    lattice_pair<0, 1, 0, 0, DEV, T>(M, w1, side, dt, pI, gO, pN, gN, invgam, rho0);
-    lattice_pair<1, 0, 1, 0, DEV, T>(M, w1, side, dt, pI, gO, pN, gN, invgam, rho0);
-    lattice_pair<2, 0, 0, 1, DEV, T>(M, w1, side, dt, pI, gO, pN, gN, invgam, rho0);
-    lattice_pair<3, 1, 1, 1, DEV, T>(M, w7, side, dt, pI, gO, pN, gN, invgam, rho0);
-    lattice_pair<4, 1, 1, -1, DEV, T>(M, w7, side, dt, pI, gO, pN, gN, invgam, rho0);
-    lattice_pair<5, 1, -1, 1, DEV, T>(M, w7, side, dt, pI, gO, pN, gN, invgam, rho0);
-    lattice_pair<6, -1, 1, 1, DEV, T>(M, w7, side, dt, pI, gO, pN, gN, invgam, rho0);
+    const T c45 = (T)4.5 * invgam, c15 = (T)1.5 * invgam;
+    T sO[4], sN[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        sO[q] = gO[3 * q] + gO[3 * q + 1] + gO[3 * q + 2];
+        sN[q] = gN[3 * q] + gN[3 * q + 1] + gN[3 * q + 2];
+    }
+    M.s0 = pop_eq<0, 0, 0, DEV, T>(w0, p0[0], p0[1], p0[2], p0[3], c45, c15, rho0);
+    lattice_pair<0, 1, 0, 0, DEV, T>(M, w1, side, dt, pI, gO, sO, pN, gN, sN, c45, c15, rho0);
+    lattice_pair<1, 0, 1, 0, DEV, T>(M, w1, side, dt, pI, gO, sO, pN, gN, sN, c45, c15, rho0);
+    lattice_pair<2, 0, 0, 1, DEV, T>(M, w1, side, dt, pI, gO, sO, pN, gN, sN, c45, c15, rho0);
+    lattice_pair<3, 1, 1, 1, DEV, T>(M, w7, side, dt, pI, gO, sO, pN, gN, sN, c45, c15, rho0);
+    lattice_pair<4, 1, 1, -1, DEV, T>(M, w7, side, dt, pI, gO, sO, pN, gN, sN, c45, c15, rho0);
+    lattice_pair<5, 1, -1, 1, DEV, T>(M, w7, side, dt, pI, gO, sO, pN, gN, sN, c45, c15, rho0);
+    lattice_pair<6, -1, 1, 1, DEV, T>(M, w7, side, dt, pI, gO, sO, pN, gN, sN, c45, c15, rho0);
     const T rI = DEV ? rho0 + M.s0 : M.s0;
     const T ir = (T)1 / rI;
     const T ux = M.sx * ir, uy = M.sy * ir, uz = M.sz * ir;
