@@ -72,13 +72,15 @@ struct DevMesh {
     const double *vol, *inv_vol, *cfl_area;
     const int32_t *blk_item_off;
     // Face work items, one record each so that an item is a few wide loads
-    // from one address.  it_x[e] = (owner, neighbour or -1-b, lattice side bits
-    // (see lattice_pair), 0).  FP64 storage: it_d[2e..2e+1] = (n, A), (dt, x_face);
-    // the centroid-to-face offsets are formed on the fly from cgeo (same bits as
-    // the host's cc - fc).  FP32 storage: it_f[3e..3e+2] = (n, A), (dt, r_owner),
-    // (r_neighbour, 0).
+    // from one address.  FP64 storage: it_x[e] = (owner, neighbour or -1-b,
+    // lattice side bits (see lattice_pair), 0) and it_d[2e..2e+1] = (n, A),
+    // (dt, x_face); the centroid-to-face offsets are formed on the fly from cgeo
+    // (same bits as the host's cc - fc).  FP32 storage (56 B per item):
+    // it_on[e] = (owner, neighbour or -1-b) and it_f[3e..3e+2] = (n, A),
+    // (dt, r_owner), (r_neighbour, side bits stored in the float's bits).
     const int4 *it_x;
     const double4 *it_d;
+    const int2 *it_on;
     const float4 *it_f;
     const double4 *cgeo;  // FP64 storage: per cell (cx, cy, cz, 1/V)
     double g0x, g0y, g0z;  // centroid of the ghost of boundary face 0

@@ -463,7 +463,7 @@ int create_ctx(const plbfs_mesh_view *mesh, const plbfs_params *params, const pl
         e.cgeo = nullptr;
         e.g0x = d.g0x; e.g0y = d.g0y; e.g0z = d.g0z;
         e.st_cnt = d.st_cnt; e.st_nbr = d.st_nbr; e.vol = d.vol; e.inv_vol = d.inv_vol; e.cfl_area = d.cfl_area;
-        e.blk_item_off = d.blk_item_off; e.it_x = d.it_x; e.g_off = d.g_off;
+        e.blk_item_off = d.blk_item_off; e.it_x = nullptr; e.g_off = d.g_off;
         e.g_ent = d.g_ent; e.b_patch = d.b_patch; e.b_partner = d.b_partner;
         auto to32 = [](const std::vector<double> &v) { return std::vector<float>(v.begin(), v.end()); };
         std::vector<DevPatch> dp32 = dp;
@@ -475,9 +475,15 @@ int create_ctx(const plbfs_mesh_view *mesh, const plbfs_params *params, const pl
             for (int q = 0; q < kGeoPerItem; q++) g[q] = (float)L.it_geo[(size_t)q * E + it];
             geo[3 * it] = float4{g[0], g[1], g[2], g[3]};
             geo[3 * it + 1] = float4{g[4], g[5], g[6], g[7]};
-            geo[3 * it + 2] = float4{g[8], g[9], g[10], 0.f};
+            float sb;
+            const uint32_t side = L.it_side[it];
+            std::memcpy(&sb, &side, sizeof sb);
+            geo[3 * it + 2] = float4{g[8], g[9], g[10], sb};
         }
+        std::vector<int2> on(E);
+        for (size_t it = 0; it < E; it++) on[it] = int2{L.it_own[it], L.it_nbr[it]};
         if ((rc = upload(c, (float **)&e.st_w, to32(L.st_w))) || (rc = upload(c, (float4 **)&e.it_f, geo)) ||
+            (rc = upload(c, (int2 **)&e.it_on, on)) ||
             (rc = upload(c, (float **)&e.b_dmag, to32(bdmag))) || (rc = upload(c, (float **)&e.corr, to32(L.corr))) ||
             (rc = upload(c, (DevPatch **)&e.patches, dp32)))
             return fail(rc);

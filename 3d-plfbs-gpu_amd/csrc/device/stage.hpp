@@ -218,12 +218,14 @@ inline size_t flux_lds_bytes(int max_items, int block_cells, bool stage, int C =
 template <typename S, typename LD>
 __device__ __forceinline__ void item_flux(const DevMesh<S> &m, const DevParams &prm, int64_t e, const LD &ld,
                                           double v3g, double dtcap, S F[4]) {
-    const int4 ix = m.it_x[e];
-    const int o = ix.x;
-    const int nb = ix.y;
-    const unsigned side = (unsigned)ix.z;
+    int o, nb;
+    unsigned side;
     S geo[kGeoPerItem];  // n(3), A, dt, r_owner(3), r_neighbour(3)
     if constexpr (std::is_same<S, double>::value) {
+        const int4 ix = m.it_x[e];
+        o = ix.x;
+        nb = ix.y;
+        side = (unsigned)ix.z;
         const double4 g0 = m.it_d[2 * e], g1 = m.it_d[2 * e + 1];
         geo[0] = g0.x; geo[1] = g0.y; geo[2] = g0.z; geo[3] = g0.w;
         geo[4] = g1.x;
@@ -243,10 +245,14 @@ __device__ __forceinline__ void item_flux(const DevMesh<S> &m, const DevParams &
             geo[10] = m.g0z - fz;
         }
     } else {
+        const int2 ix = m.it_on[e];
+        o = ix.x;
+        nb = ix.y;
         const float4 g0 = m.it_f[3 * e], g1 = m.it_f[3 * e + 1], g2 = m.it_f[3 * e + 2];
         geo[0] = g0.x; geo[1] = g0.y; geo[2] = g0.z; geo[3] = g0.w;
         geo[4] = g1.x; geo[5] = g1.y; geo[6] = g1.z; geo[7] = g1.w;
         geo[8] = g2.x; geo[9] = g2.y; geo[10] = g2.z;
+        side = __float_as_uint(g2.w);
     }
     const S n[3] = {geo[0], geo[1], geo[2]};
     const S A = geo[3], dt = fmin(geo[4], (S)dtcap);

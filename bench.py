@@ -163,6 +163,9 @@ def run_to_tolerance(args):
     FP64 at convergence.  One JSON line."""
     from plbfs import warmstart as ws
     pm, case, desc, ibm = workload(args.workload)
+    if args.re:
+        case.reynolds_no = args.re
+        desc += " (run at Re=%g)" % args.re
     if args.no_ibm:
         ibm = None
     hm = HostMesh.from_polymesh(pm, case.visc, case.gamma, case.fneq_min)
@@ -228,6 +231,7 @@ def main():
     ap.add_argument("--max-steps", type=int, default=60000)
     ap.add_argument("--params", default="", help="trained warm-start parameters (.npy); default plbfs/data")
     ap.add_argument("--probe", type=int, default=200, help="steps of the warm-start fallback probe")
+    ap.add_argument("--re", type=float, default=0.0, help="--time-to-tol: Reynolds number other than the workload's")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a device copy of 1 GiB first (PMC FETCH/WRITE calibration)")
     args = ap.parse_args()
