@@ -341,7 +341,7 @@ int retile(plbfs_batch *b, int block_cells, int C, int stage) {
     }
     plbfs_mesh_view v = view_of(b);
     int rc = create_ctx(&v, &b->params, b->patches.data(), (int32_t)b->patches.size(),
-                        b->bpatch.empty() ? nullptr : b->bpatch.data(), block_cells, C, false, &b->base);
+                        b->bpatch.empty() ? nullptr : b->bpatch.data(), block_cells, C, false, false, &b->base);
     if (rc) return rc;
     b->block_cells = block_cells;
     b->C = C;

@@ -61,6 +61,7 @@ struct plbfs_ctx {
     double *grad = nullptr;                 // 12 per cell
     double *dtl = nullptr;                  // [N]
     double *fbody = nullptr;                // [3][N] immersed-boundary force density
+    void *bflux = nullptr;                  // [4][n_border] border-face fluxes (storage precision)
     double *partials = nullptr;             // [n_blocks*5]
     double *bmin = nullptr;                 // [n_blocks]
     double *hist = nullptr;
@@ -123,10 +124,11 @@ struct TimedLaunch {
 
 // plbfs_create with an explicit flux block size (64..1024 cells, a multiple of
 // 64), LDS sized for lds_cases cases per block (the batched solver sizes its
-// own), and staging of the flux block's own records.
+// own), staging of the flux block's own records, and border faces either
+// evaluated once by a separate pass (split_border) or by both blocks.
 int create_ctx(const plbfs_mesh_view *mesh, const plbfs_params *params, const plbfs_patch *patches,
                int32_t n_patches, const int32_t *bface_patch, int block_cells, int lds_cases, bool flux_stage,
-               plbfs_ctx **out);
+               bool split_border, plbfs_ctx **out);
 
 // Launches the per-step immersed-boundary kernels (ibm.hip); no-op when off.
 int ibm_launch(plbfs_ctx *c);

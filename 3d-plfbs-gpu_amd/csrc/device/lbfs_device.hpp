@@ -61,6 +61,11 @@ __device__ __forceinline__ int xcd_block(int bid, int nblk, int remap) {
 template <typename S>
 struct DevMesh {
     int N, G, n_blocks, max_items;
+    // border items [border0, border0 + n_border) and their fluxes [4][n_border]
+    // (one evaluation per face between two blocks; 0 when items are duplicated)
+    int64_t border0;
+    int n_border;
+    S *bflux;
     int xcd_remap;
     int block_cells;  // cells per flux block (a multiple of 64)
     int stage;        // 1: flux blocks stage their own cells' records in LDS

@@ -105,6 +105,23 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
     gi[2] = V4<S>{g[8], g[9], g[10], g[11]};
 }
 
+// Faces between two flux blocks, one thread per face: flux into the border
+// array that both blocks' cells read in their gather.
+template <typename S>
+__global__ __launch_bounds__(kBlock) void k_flux_border(DevMesh<S> m, DevParams prm, const Ctrl *__restrict__ ctrl,
+                                                        const V4<S> *__restrict__ st_in,
+                                                        const V4<S> *__restrict__ grad) {
+    if (ctrl->done) return;
+    const int t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= m.n_border) return;
+    const SideLoader<S, Rec1> ld{st_in, grad, m.cgeo, Rec1{}, nullptr, nullptr, 0, 0, 1, 0};
+    S F[4];
+    item_flux<S>(m, prm, m.border0 + t, ld, prm.visc3_invgam, INFINITY, F);
+    const size_t nbd = (size_t)m.n_border;
+#pragma unroll
+    for (int q = 0; q < 4; q++) m.bflux[q * nbd + t] = F[q];
+}
+
 template <typename S>
 __global__ __launch_bounds__(kBlock) void k_flux_update(DevMesh<S> m, DevParams prm, StageArgs sa,
                                                         const Ctrl *__restrict__ ctrl,
@@ -342,12 +359,18 @@ extern "C" int plbfs_create(const plbfs_mesh_view *mesh, const plbfs_params *par
     // while staging the gradient kernel's 256 states saves 12 %; flux staging
     // is therefore opt-in (PLBFS_FLUX_STAGE=1) and gradient staging opt-out
     const bool flux_stage = env_flag("PLBFS_FLUX_STAGE", false);
-    return create_ctx(mesh, params, patches, n_patches, bface_patch, block_cells, 1, flux_stage, out);
+    // faces between two blocks: both blocks evaluate them, or, with
+    // PLBFS_SPLIT_BORDER=1, a separate border pass evaluates each once.
+    // Measured on MI355X (4.09M tets): the border pass makes the flux step
+    // slower (FP64 0.555 -> 0.617 ms, FP32 0.331 -> 0.364 ms; its faces
+    // gather both sides from global memory), so duplication is the default
+    const bool split = env_flag("PLBFS_SPLIT_BORDER", false);
+    return create_ctx(mesh, params, patches, n_patches, bface_patch, block_cells, 1, flux_stage, split, out);
 }
 
 int create_ctx(const plbfs_mesh_view *mesh, const plbfs_params *params, const plbfs_patch *patches,
                int32_t n_patches, const int32_t *bface_patch, int block_cells, int lds_cases, bool flux_stage,
-               plbfs_ctx **out) {
+               bool split_border, plbfs_ctx **out) {
     if (!mesh || !params || !out) return plbfs_fail(PLBFS_ERR_INPUT, "null argument");
     *out = nullptr;
     const plbfs_params &p = *params;
@@ -362,7 +385,7 @@ int create_ctx(const plbfs_mesh_view *mesh, const plbfs_params *params, const pl
     c->params = p;
     c->f32 = p.precision == PLBFS_FP32;
     c->patches.assign(patches, patches + n_patches);
-    int rc = build_layout(*mesh, p, patches, n_patches, bface_patch, block_cells, c->L);
+    int rc = build_layout(*mesh, p, patches, n_patches, bface_patch, block_cells, split_border, c->L);
     if (rc) {
         delete c;
         return rc;
@@ -394,6 +417,9 @@ int create_ctx(const plbfs_mesh_view *mesh, const plbfs_params *params, const pl
     d.n_blocks = L.n_blocks;
     d.max_items = L.max_items_per_block;
     d.E = L.n_items;
+    d.border0 = L.border0;
+    d.n_border = (int)L.n_border;
+    d.bflux = nullptr;
     d.block_cells = L.block_cells;
     d.stage = stage ? 1 : 0;
     d.grad_stage = env_flag("PLBFS_GRAD_STAGE", true) ? 1 : 0;
@@ -455,6 +481,7 @@ int create_ctx(const plbfs_mesh_view *mesh, const plbfs_params *params, const pl
         // patch densities become deviations from rho0
         DevMesh<float> &e = c->dm32;
         e.N = d.N; e.G = d.G; e.n_blocks = d.n_blocks; e.max_items = d.max_items; e.E = d.E;
+        e.border0 = d.border0; e.n_border = d.n_border; e.bflux = nullptr;
         e.xcd_remap = d.xcd_remap;
         e.block_cells = d.block_cells;
         e.stage = d.stage;
@@ -497,8 +524,11 @@ int create_ctx(const plbfs_mesh_view *mesh, const plbfs_params *params, const pl
         (rc = dalloc(c, (void **)&c->fbody, (size_t)3 * N * sizeof(double))) ||
         (rc = dalloc(c, (void **)&c->partials, (size_t)L.n_blocks * 5 * sizeof(double))) ||
         (rc = dalloc(c, (void **)&c->bmin, (size_t)c->cell_grid * sizeof(double))) ||
-        (rc = dalloc(c, (void **)&c->ctrl, sizeof(Ctrl))))
+        (rc = dalloc(c, (void **)&c->ctrl, sizeof(Ctrl))) ||
+        (rc = dalloc(c, &c->bflux, (size_t)4 * std::max<int64_t>(L.n_border, 1) * sbytes)))
         return fail(rc);
+    c->dm.bflux = (double *)c->bflux;
+    c->dm32.bflux = (float *)c->bflux;
     {
         std::vector<double> z((size_t)3 * N, 0.0);
         if (hipMemcpy(c->fbody, z.data(), z.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
@@ -641,6 +671,9 @@ static void launch_stages(plbfs_ctx *c, const DevMesh<S> &dm) {
         StageArgs sa{kAlpha[k], kBeta[k], k};
         {
             TimedLaunch t(c, PLBFS_KSTAT_FLUX);
+            if (dm.n_border > 0)
+                hipLaunchKernelGGL(k_flux_border<S>, dim3((dm.n_border + kBlock - 1) / kBlock), dim3(kBlock), 0,
+                                   c->stream, dm, c->prm, c->ctrl, (const V4<S> *)in, (const V4<S> *)c->grad);
             hipLaunchKernelGGL(k_flux_update<S>, dim3(nb), dim3(kBlock), c->lds_bytes, c->stream, dm, c->prm, sa,
                                c->ctrl, (const V4<S> *)in, (const V4<S> *)c->grad, (V4<S> *)outb,
                                (double4 *)c->t0, (Acc<S> *)c->t1, c->dtl, c->fbody, c->partials);
@@ -768,11 +801,19 @@ extern "C" int plbfs_eval_residual(plbfs_ctx *c, double *grads, double *R) {
     if (!c->f32) {
         hipLaunchKernelGGL(k_grad<double>, dim3(c->cell_grid), dim3(kBlock), 0, c->stream, c->dm, c->ctrl,
                            (const double4 *)c->stA, (double4 *)c->grad);
+        if (c->dm.n_border > 0)
+            hipLaunchKernelGGL(k_flux_border<double>, dim3((c->dm.n_border + kBlock - 1) / kBlock), dim3(kBlock), 0,
+                               c->stream, c->dm, c->prm, c->ctrl, (const double4 *)c->stA,
+                               (const double4 *)c->grad);
         hipLaunchKernelGGL(k_residual_only<double>, dim3(nb), dim3(kBlock), c->lds_bytes, c->stream, c->dm, c->prm,
                            (const double4 *)c->stA, (const double4 *)c->grad, c->fbody, Rd);
     } else {
         hipLaunchKernelGGL(k_grad<float>, dim3(c->cell_grid), dim3(kBlock), 0, c->stream, c->dm32, c->ctrl,
                            (const float4 *)c->stA, (float4 *)c->grad);
+        if (c->dm32.n_border > 0)
+            hipLaunchKernelGGL(k_flux_border<float>, dim3((c->dm32.n_border + kBlock - 1) / kBlock), dim3(kBlock), 0,
+                               c->stream, c->dm32, c->prm, c->ctrl, (const float4 *)c->stA,
+                               (const float4 *)c->grad);
         hipLaunchKernelGGL(k_residual_only<float>, dim3(nb), dim3(kBlock), c->lds_bytes, c->stream, c->dm32,
                            c->prm, (const float4 *)c->stA, (const float4 *)c->grad, c->fbody, Rd);
     }

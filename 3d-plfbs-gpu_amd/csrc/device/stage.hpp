@@ -321,13 +321,23 @@ __device__ __forceinline__ void gather_residual(const DevMesh<S> &m, int i, cons
     const int k0 = m.g_off[i], k1 = m.g_off[i + 1];
     for (int k = k0; k < k1; k++) {
         const int ent = m.g_ent[k];
-        const int sl = (ent >> 1) * C + c;
+        S v[4];
+        if (ent & 2) {
+            // a face between two blocks: its flux was evaluated once by the border pass
+            const size_t j = (size_t)(ent >> 2), nbd = (size_t)m.n_border;
+#pragma unroll
+            for (int q = 0; q < 4; q++) v[q] = m.bflux[q * nbd + j];
+        } else {
+            const int sl = (ent >> 2) * C + c;
+#pragma unroll
+            for (int q = 0; q < 4; q++) v[q] = sF[q * stride + sl];
+        }
         if (ent & 1) {
 #pragma unroll
-            for (int q = 0; q < 4; q++) R[q] = R[q] + (-(double)sF[q * stride + sl] * iv);
+            for (int q = 0; q < 4; q++) R[q] = R[q] + (-(double)v[q] * iv);
         } else {
 #pragma unroll
-            for (int q = 0; q < 4; q++) R[q] = R[q] + ((double)sF[q * stride + sl] * iv);
+            for (int q = 0; q < 4; q++) R[q] = R[q] + ((double)v[q] * iv);
         }
     }
     if constexpr (std::is_same<S, float>::value) {

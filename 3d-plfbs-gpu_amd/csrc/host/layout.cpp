@@ -117,7 +117,7 @@ struct NearestCell {
 };
 
 int build_layout(const plbfs_mesh_view &m, const plbfs_params &p, const plbfs_patch *patches, int n_patches,
-                 const int32_t *bface_patch, int block_cells, Layout &L) {
+                 const int32_t *bface_patch, int block_cells, bool split_border, Layout &L) {
     const int N = m.n_cells, F = m.n_faces, NI = m.n_internal_faces, G = F - NI;
     if (N <= 0 || F <= 0 || NI < 0 || NI > F || !m.owner || !m.neighbour || !m.cell_centroid || !m.cell_volume ||
         !m.face_centroid || !m.face_normal || !m.face_area || !m.face_dt)
@@ -379,8 +379,8 @@ int build_layout(const plbfs_mesh_view &m, const plbfs_params &p, const plbfs_pa
     L.g_off.assign(N + 1, 0);
     L.g_ent.clear();
     L.g_ent.reserve(s_off[N]);
-    std::vector<int32_t> mark(F, -1), slot_of(F, 0);
-    std::vector<int32_t> item_face;
+    std::vector<int32_t> mark(F, -1), slot_of(F, 0), border_of(F, -1);
+    std::vector<int32_t> item_face, border_face;
     item_face.reserve((size_t)F + F / 4);
     int max_items = 0;
     for (int blk = 0; blk < L.n_blocks; blk++) {
@@ -391,12 +391,21 @@ int build_layout(const plbfs_mesh_view &m, const plbfs_params &p, const plbfs_pa
             for (int s = s_off[i]; s < s_off[i + 1]; s++) {
                 const int f = s_face[s];
                 if (f > NI && patches[bface_patch[f - NI]].is_empty) continue;  // Solver.cpp:308
+                const int own = m.owner[f] == i ? 1 : 0;
+                if (split_border && f < NI && L.inv[m.owner[f]] / C != L.inv[m.neighbour[f]] / C) {
+                    if (border_of[f] < 0) {
+                        border_of[f] = (int32_t)border_face.size();
+                        border_face.push_back(f);
+                    }
+                    L.g_ent.push_back((border_of[f] << 2) | 2 | own);
+                    continue;
+                }
                 if (mark[f] != blk) {
                     mark[f] = blk;
                     slot_of[f] = (int)item_face.size() - first_item;
                     item_face.push_back(f);
                 }
-                L.g_ent.push_back((slot_of[f] << 1) | (m.owner[f] == i ? 1 : 0));
+                L.g_ent.push_back((slot_of[f] << 2) | own);
             }
             L.g_off[n + 1] = (int32_t)L.g_ent.size();
         }
@@ -404,6 +413,9 @@ int build_layout(const plbfs_mesh_view &m, const plbfs_params &p, const plbfs_pa
         max_items = std::max(max_items, (int)item_face.size() - first_item);
     }
     L.max_items_per_block = max_items;
+    L.border0 = (int64_t)item_face.size();
+    L.n_border = (int64_t)border_face.size();
+    item_face.insert(item_face.end(), border_face.begin(), border_face.end());
     const size_t E = item_face.size();
     L.n_items = (int64_t)E;
     L.it_own.resize(E);

@@ -7,10 +7,12 @@
 // and its patch, evaluated where needed.
 //
 // Face work is organised per block of `block_cells` consecutive cells: every
-// face with at least one cell in the block becomes a work item of that block
-// (faces on the border of two blocks are evaluated by both), its flux goes to
-// a block-local LDS slot, and each cell then sums its own faces' slots in the
-// reference's face order (the order of src/Solver.cpp:286-366).  No float
+// face with at least one cell in the block becomes a work item of that block,
+// its flux goes to a block-local LDS slot, and each cell then sums its own
+// faces' slots in the reference's face order (the order of
+// src/Solver.cpp:286-366).  A face on the border of two blocks is either
+// evaluated by both (duplicated items), or, with split_border, once by a
+// separate border pass into a global array that both blocks read.  No float
 // atomics are involved.
 #pragma once
 #include <cstdint>
@@ -29,6 +31,9 @@ struct Layout {
     int n_blocks = 0;
     int max_items_per_block = 0;
     int64_t n_items = 0;
+    // border items (split_border): items [border0, border0 + n_border) are the
+    // faces between two blocks, each evaluated once into a global flux array
+    int64_t border0 = 0, n_border = 0;
     std::vector<int32_t> perm;  // new cell -> file cell
     std::vector<int32_t> inv;   // file cell -> new cell
     // LSQ stencil, slot-major: element (s, i) at s*N + i
@@ -53,7 +58,8 @@ struct Layout {
     std::vector<double> it_geo;  // [kGeoPerItem*E], component-major
     std::vector<double> it_fc;   // [3*E] face centroid, component-major
     double ghost0[3] = {0, 0, 0};  // centroid of the ghost of boundary face 0
-    // per-cell gather list, entries (slot << 1) | is_owner in file face order
+    // per-cell gather list in file face order, entries (index << 2) | border << 1 | is_owner:
+    // index is the block-local slot, or with the border bit the border item's index
     std::vector<int32_t> g_off;  // [N+1]
     std::vector<int32_t> g_ent;
     // boundary faces (b = face - NI)
@@ -68,7 +74,7 @@ struct Layout {
 
 // Builds the layout.  Returns PLBFS_OK or an error code (message via plbfs_fail).
 int build_layout(const plbfs_mesh_view &m, const plbfs_params &p, const plbfs_patch *patches, int n_patches,
-                 const int32_t *bface_patch, int block_cells, Layout &L);
+                 const int32_t *bface_patch, int block_cells, bool split_border, Layout &L);
 
 // 3-D Hilbert key of integer coordinates with `bits` bits per axis.
 uint64_t hilbert_key3(uint32_t x, uint32_t y, uint32_t z, int bits);

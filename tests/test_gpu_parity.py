@@ -144,6 +144,30 @@ def test_flux_block_sizes(gpu_available, monkeypatch, block_cells, stage):
     assert max(errs) < 1e-10, errs
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+def test_border_pass_is_bitwise_neutral(gpu_available, monkeypatch, precision):
+    # faces between two flux blocks evaluated once by the border pass or by
+    # both blocks: the same arithmetic summed in the same order, so the same bits
+    pm = channel_mesh(16, 8, 8, tets=True)
+    case = pc.channel_case(re=40.0, length_scale=6.0)
+    case.lsq_exact_det = 1
+    case.precision = precision
+    st = None
+    out = []
+    for split in ("1", "0"):
+        monkeypatch.setenv("PLBFS_SPLIT_BORDER", split)
+        monkeypatch.setenv("PLBFS_BLOCK_CELLS", "64")
+        o, hm, s = make_pair(pm, case)
+        if st is None:
+            st = perturbed_state(case, o, amp=0.01, seed=9)
+        s.set_state(*st)
+        assert s.run(30)["rc"] == 0
+        out.append(s.state())
+        s.close()
+    for q in range(4):
+        assert np.array_equal(out[0][q], out[1][q]), q
+
+
 def test_reference_determinant_on_jittered_tets(gpu_available):
     """The reference's LSQ determinant on jittered tets: both sides follow it
     for the few steps before the reference algorithm itself blows up."""
