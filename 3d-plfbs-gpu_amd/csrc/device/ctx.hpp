@@ -39,6 +39,7 @@ struct IbmState {
     int32_t *np_off = nullptr, *np_cell = nullptr;           // node -> candidate cells
     int32_t *cp_cell = nullptr, *cp_off = nullptr, *cp_node = nullptr;  // affected cell -> nodes
     int32_t *cp_pair = nullptr;  // node-major pair index of each cell-major entry
+    int32_t *blk_force = nullptr;  // [n_blocks] 1 where a flux block holds an affected cell
     double *pw = nullptr;        // [n_pairs] kernel weights at the current node positions
     int n_affected = 0;
     int64_t n_pairs = 0;

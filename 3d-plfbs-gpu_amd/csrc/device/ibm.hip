@@ -201,6 +201,14 @@ extern "C" int plbfs_set_ibm(plbfs_ctx *c, const plbfs_ibm_nodes *nodes) {
         (rc = upload(c, &s.cell_c, c->L.centroid)) || (rc = dalloc(c, (void **)&s.pw, np_cell.size() * sizeof(double))))
         return rc;
     {
+        // flux blocks that hold an affected cell: only these read the body force
+        std::vector<int32_t> bf((size_t)c->L.n_blocks, 0);
+        for (int32_t cell : cp_cell) bf[(size_t)cell / c->L.block_cells] = 1;
+        if ((rc = upload(c, &s.blk_force, bf))) return rc;
+        c->dm.blk_force = s.blk_force;
+        c->dm32.blk_force = s.blk_force;
+    }
+    {
         const IbmArgs a = make_args(c);
         hipLaunchKernelGGL(k_ibm_weights, dim3((n + kNodesPerBlock - 1) / kNodesPerBlock), dim3(kIbmThreads), 0,
                            c->stream, a);

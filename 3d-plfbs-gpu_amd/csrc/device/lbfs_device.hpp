@@ -90,6 +90,9 @@ struct DevMesh {
     const double4 *cgeo;  // FP64 storage: per cell (cx, cy, cz, 1/V)
     double g0x, g0y, g0z;  // centroid of the ghost of boundary face 0
     const int32_t *g_off, *g_ent;
+    // [n_blocks] nonzero where a flux block holds a cell the immersed boundary
+    // spreads force to; null: every block reads the body force
+    const int32_t *blk_force;
     const int32_t *b_patch;
     const int32_t *b_partner;  // periodic partner cell of each boundary face, -1 elsewhere
     const S *b_dmag;

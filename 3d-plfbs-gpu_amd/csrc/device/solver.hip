@@ -152,7 +152,7 @@ __global__ __launch_bounds__(kBlock) void k_flux_update(DevMesh<S> m, DevParams 
     for (int i = blk * m.block_cells + threadIdx.x; i < c_end; i += kBlock) {
         double R[4];
         gather_residual<S>(m, i, sF, nslot, 1, 0, R);
-        if (prm.use_body_force) {
+        if (prm.use_body_force && (!m.blk_force || m.blk_force[blk])) {
             R[1] += fbody[i];
             R[2] += fbody[(size_t)N + i];
             R[3] += fbody[2 * (size_t)N + i];
@@ -420,6 +420,7 @@ int create_ctx(const plbfs_mesh_view *mesh, const plbfs_params *params, const pl
     d.border0 = L.border0;
     d.n_border = (int)L.n_border;
     d.bflux = nullptr;
+    d.blk_force = nullptr;
     d.block_cells = L.block_cells;
     d.stage = stage ? 1 : 0;
     d.grad_stage = env_flag("PLBFS_GRAD_STAGE", true) ? 1 : 0;
@@ -481,7 +482,7 @@ int create_ctx(const plbfs_mesh_view *mesh, const plbfs_params *params, const pl
         // patch densities become deviations from rho0
         DevMesh<float> &e = c->dm32;
         e.N = d.N; e.G = d.G; e.n_blocks = d.n_blocks; e.max_items = d.max_items; e.E = d.E;
-        e.border0 = d.border0; e.n_border = d.n_border; e.bflux = nullptr;
+        e.border0 = d.border0; e.n_border = d.n_border; e.bflux = nullptr; e.blk_force = nullptr;
         e.xcd_remap = d.xcd_remap;
         e.block_cells = d.block_cells;
         e.stage = d.stage;
