@@ -41,6 +41,7 @@ struct BatchArgs {
     const double *v3g;      // [B] 3 nu / gamma
     const double *dtcap;    // [B] streaming dt cap 3 nu / (gamma fneq_min), or infinity
     const int32_t *done;    // [B] frozen cases
+    const uint8_t *force_rec;  // [N*B] 1 where a (cell, case) record receives immersed-boundary force
 };
 
 // Per-case control written by kb_finalize.
@@ -117,7 +118,7 @@ __global__ __launch_bounds__(kBlock) void kb_flux(DevMesh<S> m, DevParams prm, B
         double R[4];
         gather_residual<S>(m, i, sF, stride, C, c, R);
         const size_t r = (size_t)i * B + b;
-        if (prm.use_body_force) {
+        if (prm.use_body_force && a.force_rec[r]) {
             R[1] += fbody[r];
             R[2] += fbody[NR + r];
             R[3] += fbody[2 * NR + r];
@@ -278,6 +279,7 @@ struct plbfs_batch {
     double ibm_h = 1, ibm_dt = 0, ibm_margin = 0;
     IbmArgs ia{};
     std::vector<void *> ibm_allocs;
+    uint8_t *force_rec = nullptr;  // in ibm_allocs
     // graph of one step
     hipGraphExec_t graph = nullptr;
     bool graph_ok = false;
@@ -359,7 +361,7 @@ int retile(plbfs_batch *b, int block_cells, int C, int stage) {
 }
 
 BatchArgs args_of(plbfs_batch *b) {
-    return BatchArgs{b->B, b->C, b->stage, b->d_visc, b->d_v3g, b->d_dtcap, b->done};
+    return BatchArgs{b->B, b->C, b->stage, b->d_visc, b->d_v3g, b->d_dtcap, b->done, b->force_rec};
 }
 
 int ibm_upload(plbfs_batch *b) {
@@ -368,6 +370,7 @@ int ibm_upload(plbfs_batch *b) {
     drop_graph(b);
     HIP_TRY(hipStreamSynchronize(b->base->stream));
     free_list(b->ibm_allocs);
+    b->force_rec = nullptr;
     std::vector<void *> &M = b->ibm_allocs;
     std::vector<double> xr, xx, kda;
     std::vector<int32_t> ncase, np_off(1, 0), np_cell, cp_cell, cp_case, cp_off(1, 0), cp_node, cp_pair;
@@ -448,6 +451,12 @@ int ibm_upload(plbfs_batch *b) {
     a.vol = b->base->dm.vol;
     a.case_done = b->done;
     a.case_status = b->status;
+    {
+        // records the spreading writes: only these read the body force
+        std::vector<uint8_t> fr((size_t)b->N * b->B, 0);
+        for (size_t j = 0; j < cp_cell.size(); j++) fr[(size_t)cp_cell[j] * b->B + cp_case[j]] = 1;
+        if ((rc = bupload(M, &b->force_rec, fr))) return rc;
+    }
     HIP_TRY(hipMemset(b->fbody, 0, (size_t)3 * b->N * b->B * sizeof(double)));
     hipLaunchKernelGGL(k_ibm_weights, dim3((n_total + kNodesPerBlock - 1) / kNodesPerBlock), dim3(kIbmThreads), 0,
                        b->base->stream, a);
