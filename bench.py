@@ -181,6 +181,8 @@ def run_to_tolerance(args, emit=True):
     out, states = {}, {}
     params = ws.load_params(args.params or None) if draw is not None else None
     legs = [("fp64", 0, False), ("fp32", 1, False)] + ([("fp32_warm", 1, True)] if params is not None else [])
+    if getattr(args, "legs", ""):
+        legs = [leg for leg in legs if leg[0] in args.legs.split(",")]
     for name, prec, warm in legs:
         case.tolerance = args.tol
         case.check_interval = 100
@@ -200,15 +202,16 @@ def run_to_tolerance(args, emit=True):
         print("%s: %d steps, converged %s, %.1f s" % (name, r["steps"], r["converged"], r["seconds"]),
               file=sys.stderr, flush=True)
         states[name] = state
-    u64 = np.stack(states["fp64"][1:], 1)
-    for name in states:
-        if name != "fp64":
-            u = np.stack(states[name][1:], 1)
-            out[name]["velocity_rel_l2_vs_fp64"] = float(np.linalg.norm(u - u64) / np.linalg.norm(u64))
-    if "fp32_warm" in out and out["fp32_warm"]["converged"] and out["fp64"]["converged"]:
-        out["speedup_fp64_rest_over_fp32_warm"] = out["fp64"]["seconds"] / out["fp32_warm"]["seconds"]
-    if out["fp32"]["converged"] and out["fp64"]["converged"]:
-        out["speedup_fp64_rest_over_fp32_rest"] = out["fp64"]["seconds"] / out["fp32"]["seconds"]
+    if "fp64" in states:
+        u64 = np.stack(states["fp64"][1:], 1)
+        for name in states:
+            if name != "fp64":
+                u = np.stack(states[name][1:], 1)
+                out[name]["velocity_rel_l2_vs_fp64"] = float(np.linalg.norm(u - u64) / np.linalg.norm(u64))
+        if "fp32_warm" in out and out["fp32_warm"]["converged"] and out["fp64"]["converged"]:
+            out["speedup_fp64_rest_over_fp32_warm"] = out["fp64"]["seconds"] / out["fp32_warm"]["seconds"]
+        if "fp32" in out and out["fp32"]["converged"] and out["fp64"]["converged"]:
+            out["speedup_fp64_rest_over_fp32_rest"] = out["fp64"]["seconds"] / out["fp32"]["seconds"]
     out["workload"] = desc
     out["tolerance"] = args.tol
     if emit:
@@ -234,6 +237,7 @@ def main():
     ap.add_argument("--params", default="", help="trained warm-start parameters (.npy); default plbfs/data")
     ap.add_argument("--probe", type=int, default=200, help="steps of the warm-start fallback probe")
     ap.add_argument("--re", type=float, default=0.0, help="--time-to-tol: Reynolds number other than the workload's")
+    ap.add_argument("--legs", default="", help="--time-to-tol: subset of fp64,fp32,fp32_warm to run")
     ap.add_argument("--no-ttt", action="store_true",
                     help="throughput only: skip the time-to-1e-6 comparison on the 96x48x48 channel")
     ap.add_argument("--copy-calibration", action="store_true",
