@@ -90,7 +90,9 @@ int build_ibm_pairs(const plbfs::Layout &L, const plbfs_ibm_nodes *nodes, IbmPai
     const int N = L.N;
     const double h = nodes->h;
     const double support = (nodes->kernel == PLBFS_DELTA_HAT2 ? 1.0 : 2.0) * h;
-    const double margin = 0.5 * h;
+    // drift margin: the penalty nodes creep slowly downstream (by ~0.5 h over
+    // 2e5 steps on the 4.1M-tet sphere case, which stopped at a 0.5 h margin)
+    const double margin = 1.0 * h;
     const double reach = support + margin;
     P.margin = margin;
     const double *cc = L.centroid.data();
