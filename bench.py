@@ -352,7 +352,7 @@ def main():
     if not args.no_ttt:
         # the second half of the metric: wall time to residual 1e-6, FP64 from
         # rest against FP32 from the learned warm start, on the 96x48x48 hex
-        # channel (the 4.1M-tet case does not reach 1e-6 within a bench run)
+        # channel (on the 4.1M-tet case the three solves take ~30 min together)
         s.close()
         t_args = argparse.Namespace(**vars(args))
         t_args.workload, t_args.re, t_args.no_ibm = "chan96", 0.0, False
