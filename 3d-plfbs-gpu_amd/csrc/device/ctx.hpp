@@ -54,6 +54,7 @@ struct plbfs_ctx {
     std::vector<plbfs_patch> patches;
     plbfs::DevParams prm{};
     std::vector<void *> allocs;
+    std::vector<void *> ibm_allocs;  // the immersed-boundary arrays, freed when plbfs_set_ibm replaces them
     bool f32 = false;                 // FP32 storage of fields, gradients and face data
     plbfs::DevMesh<double> dm{};      // FP64 storage (also holds the shared int/FP64 arrays)
     plbfs::DevMesh<float> dm32{};     // FP32 storage
@@ -87,19 +88,26 @@ struct plbfs_ctx {
     bool use_graph = true;
 };
 
-inline int dalloc(plbfs_ctx *c, void **p, size_t bytes) {
+inline int dalloc(std::vector<void *> &list, void **p, size_t bytes) {
     if (bytes == 0) bytes = 8;
     HIP_TRY(hipMalloc(p, bytes));
-    c->allocs.push_back(*p);
+    list.push_back(*p);
+    return PLBFS_OK;
+}
+
+inline int dalloc(plbfs_ctx *c, void **p, size_t bytes) { return dalloc(c->allocs, p, bytes); }
+
+template <typename T>
+inline int upload(std::vector<void *> &list, T **dst, const std::vector<T> &src) {
+    int rc = dalloc(list, (void **)dst, src.size() * sizeof(T));
+    if (rc) return rc;
+    if (!src.empty()) HIP_TRY(hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
     return PLBFS_OK;
 }
 
 template <typename T>
 inline int upload(plbfs_ctx *c, T **dst, const std::vector<T> &src) {
-    int rc = dalloc(c, (void **)dst, src.size() * sizeof(T));
-    if (rc) return rc;
-    if (!src.empty()) HIP_TRY(hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
-    return PLBFS_OK;
+    return upload(c->allocs, dst, src);
 }
 
 // Brackets one launch with events on the context's stream (timing mode only);

@@ -177,7 +177,21 @@ extern "C" int plbfs_set_ibm(plbfs_ctx *c, const plbfs_ibm_nodes *nodes) {
     const double margin = P.margin;
     const std::vector<int32_t> &np_off = P.np_off, &np_cell = P.np_cell, &cp_cell = P.cp_cell, &cp_off = P.cp_off,
                                &cp_node = P.cp_node;
-    // device arrays
+    // device arrays; the previous node set's arrays are freed once no queued
+    // work can read them (the captured step graph that references them is
+    // dropped below)
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (c->graph_exec) {
+        (void)hipGraphExecDestroy(c->graph_exec);
+        c->graph_exec = nullptr;
+    }
+    c->graph_ok = false;
+    for (void *p : c->ibm_allocs) (void)hipFree(p);
+    c->ibm_allocs.clear();
+    c->dm.blk_force = nullptr;
+    c->dm32.blk_force = nullptr;
+    c->prm.use_body_force = 0;  // until the new node set is complete
+    std::vector<void *> &M = c->ibm_allocs;
     IbmState &s = c->ibm;
     s = IbmState();
     s.n = n;
@@ -195,18 +209,18 @@ extern "C" int plbfs_set_ibm(plbfs_ctx *c, const plbfs_ibm_nodes *nodes) {
             xr[(size_t)q * n + k] = nodes->x_ref[3 * k + q];
             xx[(size_t)q * n + k] = nodes->x[3 * k + q];
         }
-    if ((rc = upload(c, &s.xref, xr)) || (rc = upload(c, &s.x, xx)) || (rc = upload(c, &s.x0, xx)) ||
-        (rc = upload(c, &s.vel, z)) || (rc = upload(c, &s.force, z)) || (rc = upload(c, &s.norm, zn)) ||
-        (rc = upload(c, &s.np_off, np_off)) || (rc = upload(c, &s.np_cell, np_cell)) ||
-        (rc = upload(c, &s.cp_cell, cp_cell)) || (rc = upload(c, &s.cp_off, cp_off)) ||
-        (rc = upload(c, &s.cp_node, cp_node)) || (rc = upload(c, &s.cp_pair, P.cp_pair)) ||
-        (rc = upload(c, &s.cell_c, c->L.centroid)) || (rc = dalloc(c, (void **)&s.pw, np_cell.size() * sizeof(double))))
+    if ((rc = upload(M, &s.xref, xr)) || (rc = upload(M, &s.x, xx)) || (rc = upload(M, &s.x0, xx)) ||
+        (rc = upload(M, &s.vel, z)) || (rc = upload(M, &s.force, z)) || (rc = upload(M, &s.norm, zn)) ||
+        (rc = upload(M, &s.np_off, np_off)) || (rc = upload(M, &s.np_cell, np_cell)) ||
+        (rc = upload(M, &s.cp_cell, cp_cell)) || (rc = upload(M, &s.cp_off, cp_off)) ||
+        (rc = upload(M, &s.cp_node, cp_node)) || (rc = upload(M, &s.cp_pair, P.cp_pair)) ||
+        (rc = upload(M, &s.cell_c, c->L.centroid)) || (rc = dalloc(M, (void **)&s.pw, np_cell.size() * sizeof(double))))
         return rc;
     {
         // flux blocks that hold an affected cell: only these read the body force
         std::vector<int32_t> bf((size_t)c->L.n_blocks, 0);
         for (int32_t cell : cp_cell) bf[(size_t)cell / c->L.block_cells] = 1;
-        if ((rc = upload(c, &s.blk_force, bf))) return rc;
+        if ((rc = upload(M, &s.blk_force, bf))) return rc;
         c->dm.blk_force = s.blk_force;
         c->dm32.blk_force = s.blk_force;
     }

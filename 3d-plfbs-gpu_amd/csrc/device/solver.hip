@@ -327,6 +327,8 @@ static int release(plbfs_ctx *c) {
     }
     for (void *p : c->allocs) hipFree(p);
     c->allocs.clear();
+    for (void *p : c->ibm_allocs) hipFree(p);
+    c->ibm_allocs.clear();
     if (c->stream) {
         hipStreamDestroy(c->stream);
         c->stream = nullptr;
