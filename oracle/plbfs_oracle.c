@@ -44,13 +44,15 @@
  *                      body force adds dt*f per step;
  *   IBM                hat weights equal the reference's structured formula,
  *                      linear fields reproduced, force conservation, adjointness,
- *                      node motion X += U dt.
+ *                      node motion X += U dt;
+ *   update_bcs         ghost states of types 1 and 2 (velocity and density
+ *                      roles): the uniform free stream is steady in the channel.
  * Parity unpinned (restated line by line from the cited source; no vector or
  * closed form distinguishes them from a plausible alternative): the value of
  * the reference LSQ determinant expression on non-axis-aligned stencils, the
  * "neighbour > n_cells" and "face > n_internal" boundary tests, the ghost
  * gradients of types 2/6/7/8 (incl. the Neumann (1-n) factor without abs), and
- * the ghost states of types 2/6/7/8.
+ * the ghost states of types 6/7/8.
  */
 #include <math.h>
 #include <stdint.h>

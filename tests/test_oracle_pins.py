@@ -329,3 +329,27 @@ def test_periodic_z_residual_is_z_independent():
             assert abs(z[p] - (nz - z[own])) < 1e-9
     # and the field is not trivially at rest
     assert np.abs(R[:, 1]).max() > 1e-6
+
+
+@pytest.mark.parametrize("fneq_min", [0.0, 0.55])
+def test_free_stream_is_steady_in_the_channel(fneq_min):
+    # uniform inflow state everywhere: the ghosts of the Dirichlet inlet and
+    # sides (velocity 2U - u = U, density Neumann rho), of the outlet (velocity
+    # Neumann u, density Dirichlet 2 rho_b - rho = rho_b) and their gradients
+    # all reproduce the free stream, so every residual vanishes; a wrong sign
+    # in any of these ghost formulas leaves a residual at that patch
+    from helpers import channel_mesh
+    pm = channel_mesh(10, 5, 5, tets=True)
+    case = pc.channel_case(re=40.0, length_scale=4.0, fneq_min=fneq_min)
+    case.lsq_exact_det = 1
+    o = oracle_for(pm, case)
+    cells = o.cells()
+    st = case.initial_state(cells["cx"], cells["cy"], cells["cz"])
+    assert np.allclose(st[1], case.mach_no) and np.allclose(st[0], 1.0)
+    R, g = o.eval_residual(*st)
+    assert np.abs(R).max() < 1e-14, np.abs(R).max()
+    assert np.abs(g).max() < 1e-12
+    # and the same state with the inflow speed doubled is not steady at the inlet
+    st2 = [st[0], 2 * st[1], st[2], st[3]]
+    R2, _ = o.eval_residual(*st2)
+    assert np.abs(R2).max() > 1e-6
