@@ -46,13 +46,15 @@
  *                      linear fields reproduced, force conservation, adjointness,
  *                      node motion X += U dt;
  *   update_bcs         ghost states of types 1 and 2 (velocity and density
- *                      roles): the uniform free stream is steady in the channel.
+ *                      roles): the uniform free stream is steady in the channel;
+ *   ghost gradients    type 2 keeps the sign of n in (1 - n): a field mirror-
+ *                      symmetric about a cube's mid-plane gets a residual that
+ *                      is symmetric except in the cells on the two Neumann walls.
  * Parity unpinned (restated line by line from the cited source; no vector or
  * closed form distinguishes them from a plausible alternative): the value of
  * the reference LSQ determinant expression on non-axis-aligned stencils, the
- * "neighbour > n_cells" and "face > n_internal" boundary tests, the ghost
- * gradients of types 2/6/7/8 (incl. the Neumann (1-n) factor without abs), and
- * the ghost states of types 6/7/8.
+ * "neighbour > n_cells" and "face > n_internal" boundary tests, and the ghost
+ * states and gradients of types 6/7/8.
  */
 #include <math.h>
 #include <stdint.h>

@@ -353,3 +353,32 @@ def test_free_stream_is_steady_in_the_channel(fneq_min):
     st2 = [st[0], 2 * st[1], st[2], st[3]]
     R2, _ = o.eval_residual(*st2)
     assert np.abs(R2).max() > 1e-6
+
+
+def test_neumann_ghost_gradient_keeps_the_sign_of_n():
+    # Neumann ghost gradients are grad * (1 - n) without abs (gradients.cpp:592-648):
+    # on the x = 0 wall (n = -x) the normal component doubles, on x = 1 (n = +x)
+    # it vanishes.  A tangential velocity mirror-symmetric about x = 1/2
+    # therefore gets a residual that is mirror-symmetric away from the x walls
+    # but not in the cells on them; (1 - |n|) would keep it symmetric everywhere.
+    pm = cavity_mesh(6, tets=False, jitter=0.0)
+    o = oracle_for(pm, allneumann_case())
+    c = o.cells()
+    N = o.N
+    x = c["cx"]
+    # v (a y velocity) is unchanged by the mirror x -> 1 - x, so its residual
+    # mirrors too wherever the boundary treatment is symmetric
+    st = [np.full(o.T, 1.0), np.zeros(o.T), 0.01 + 0.4 * (x - 0.5) ** 2, np.zeros(o.T)]
+    R, g = o.eval_residual(*st)
+    xyz = np.stack([c["cx"][:N], c["cy"][:N], c["cz"][:N]], 1)
+    mx = xyz.copy()
+    mx[:, 0] = 1.0 - mx[:, 0]
+    dist = np.linalg.norm(xyz[None, :, :] - mx[:, None, :], axis=2)
+    mirror = dist.argmin(1)
+    assert dist.min(1).max() < 1e-6  # float32 points: centroids mirror to ~1e-8
+    h = 1.0 / 6
+    wall = (xyz[:, 0] < h) | (xyz[:, 0] > 1 - h)
+    d = np.abs(R[:, 2] - R[mirror, 2])
+    scale = np.abs(R[:, 2]).max()
+    assert d[~wall].max() < 1e-6 * scale  # float32 points: the mesh mirrors to ~1e-8
+    assert d[wall].max() > 1e-3 * scale
