@@ -45,16 +45,19 @@
  *   IBM                hat weights equal the reference's structured formula,
  *                      linear fields reproduced, force conservation, adjointness,
  *                      node motion X += U dt;
- *   update_bcs         ghost states of types 1 and 2 (velocity and density
- *                      roles): the uniform free stream is steady in the channel;
+ *   update_bcs         ghost states of types 1, 2, 6 and 7: the uniform free
+ *                      stream is steady in the channel whichever of them the
+ *                      inlet uses, and a density jump against the inlet patch
+ *                      leaves a residual there for 6 and 7 (patch value) but
+ *                      not for 2 (owner value);
  *   ghost gradients    type 2 keeps the sign of n in (1 - n): a field mirror-
  *                      symmetric about a cube's mid-plane gets a residual that
  *                      is symmetric except in the cells on the two Neumann walls.
  * Parity unpinned (restated line by line from the cited source; no vector or
  * closed form distinguishes them from a plausible alternative): the value of
  * the reference LSQ determinant expression on non-axis-aligned stencils, the
- * "neighbour > n_cells" and "face > n_internal" boundary tests, and the ghost
- * states and gradients of types 6/7/8.
+ * "neighbour > n_cells" and "face > n_internal" boundary tests, the ghost
+ * state of type 8 and the ghost gradients of types 6/7/8.
  */
 #include <math.h>
 #include <stdint.h>

@@ -382,3 +382,32 @@ def test_neumann_ghost_gradient_keeps_the_sign_of_n():
     scale = np.abs(R[:, 2]).max()
     assert d[~wall].max() < 1e-6 * scale  # float32 points: the mesh mirrors to ~1e-8
     assert d[wall].max() > 1e-3 * scale
+
+
+@pytest.mark.parametrize("inlet_vel,inlet_rho", [("dirichlet", "neumann"), ("inlet", "inlet"), ("inlet", "wall")])
+def test_ghost_states_take_the_patch_or_the_owner_value(inlet_vel, inlet_rho):
+    # free stream U with density rho_i inside; the inlet patch holds U and rho = 1.
+    # Velocity types 1 (2U_b - u) and 6 (U_b) both reproduce U; density type 2
+    # copies the owner (steady for any rho_i), types 6 and 7 impose the patch
+    # density (steady only for rho_i = 1)
+    from helpers import channel_mesh
+    pm = channel_mesh(10, 5, 5, tets=True)
+    case = pc.channel_case(re=40.0, length_scale=4.0)
+    case.bcs["inlet"] = dict(rho=1.0, u=1.0, v=0.0, w=0.0, vel_type=inlet_vel, rho_type=inlet_rho)
+    case.lsq_exact_det = 1
+    o = oracle_for(pm, case)
+    cells = o.cells()
+    st = case.initial_state(cells["cx"], cells["cy"], cells["cz"])
+    R, _ = o.eval_residual(*st)
+    assert np.abs(R).max() < 1e-14
+    st_r = [np.full(o.T, 1.01), st[1], st[2], st[3]]
+    R_r, _ = o.eval_residual(*st_r)
+    near_outlet_or_inlet = np.abs(R_r).max()
+    if inlet_rho == "neumann":
+        # the outlet imposes rho = 1 (type 1): only its cells feel the jump
+        x = cells["cx"][: o.N]
+        assert np.abs(R_r[x < 0.5 * x.max()]).max() < 1e-14
+    else:
+        x = cells["cx"][: o.N]
+        assert np.abs(R_r[x < 0.2 * x.max()]).max() > 1e-6
+    assert near_outlet_or_inlet > 1e-6
