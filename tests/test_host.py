@@ -305,3 +305,31 @@ def test_shipped_warm_start_parameters():
     assert p is not None, "plbfs/data/mpnn_channel.npy missing"
     assert p.dtype == np.float32 and p.shape == (nat.lib().plbfs_mpnn_n_params(),)
     assert np.all(np.isfinite(p)) and 0 < np.abs(p).max() < 100
+
+
+def test_warm_start_training_schedule_and_field_filter():
+    from plbfs import warmstart as ws
+
+    class FakeNet:
+        def __init__(self):
+            self.calls = []
+
+        def train(self, X, T, mask, epochs, lr, seed):
+            self.calls.append((epochs, lr, seed))
+            return np.array([1.0 / (len(self.calls))])
+
+        def params(self):
+            return np.zeros(3, np.float32)
+
+    net = FakeNet()
+    hist = ws.train_epochs(net, None, None, None, epochs=5, lr=1e-3, seed=7)
+    lrs = [c[1] for c in net.calls]
+    assert [c[0] for c in net.calls] == [1] * 5 and [c[2] for c in net.calls] == [7, 8, 9, 10, 11]
+    assert lrs[0] == pytest.approx(1e-3) and lrs[-1] == pytest.approx(5e-5)
+    assert all(a > b for a, b in zip(lrs, lrs[1:]))          # cosine decay, monotone
+    assert lrs[2] == pytest.approx(0.5 * (1e-3 + 5e-5))     # half-way point of the cosine
+    assert np.allclose(hist, [1, 1 / 2, 1 / 3, 1 / 4, 1 / 5])
+    ok = dict(status=0, residual=[0, 2e-4, 1e-4, 1e-4], converged=False)
+    assert ws.usable(ok)
+    assert not ws.usable(dict(ok, status=-3))
+    assert not ws.usable(dict(ok, residual=[0, 5e-3, 0, 0]))
