@@ -61,6 +61,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
     if (i >= m.N || a.done[b]) return;
     S g[12];
     lsq_grad<S>(m, i, GlobalStates<S, RecB>{st, RecB{a.B, b}}, g);
+    // the batch keeps the state apart: three gradient V4 per (cell, case) record
     V4<S> *gi = grad + 3 * ((size_t)i * a.B + b);
     gi[0] = V4<S>{g[0], g[1], g[2], g[3]};
     gi[1] = V4<S>{g[4], g[5], g[6], g[7]};
@@ -91,7 +92,7 @@ __global__ __launch_bounds__(kBlock) void kb_flux(DevMesh<S> m, DevParams prm, B
         sRec = reinterpret_cast<V4<S> *>(base);
         if (std::is_same<S, double>::value)
             sCg = reinterpret_cast<double4 *>(base + (size_t)m.block_cells * C * 4 * sizeof(V4<S>));
-        stage_records<S>(st_in, grad, std::is_same<S, double>::value ? m.cgeo : nullptr, B, b0, C, c0, ncell, sRec,
+        stage_records<S, false>(st_in, grad, std::is_same<S, double>::value ? m.cgeo : nullptr, B, b0, C, c0, ncell, sRec,
                          sCg);
         nst = ncell;
         __syncthreads();
@@ -101,7 +102,7 @@ __global__ __launch_bounds__(kBlock) void kb_flux(DevMesh<S> m, DevParams prm, B
         const int el = k / C, c = k - el * C, b = b0 + c;
         if (b >= B || a.done[b]) continue;
         S F[4];
-        const SideLoader<S, RecB> ld{st_in, grad, m.cgeo, RecB{B, b}, sRec, sCg, c0, nst, C, c};
+        const SideLoader<S, RecB, false> ld{st_in, grad, m.cgeo, RecB{B, b}, sRec, sCg, c0, nst, C, c};
         item_flux<S>(m, prm, e0 + el, ld, a.v3g[b], a.dtcap[b], F);
 #pragma unroll
         for (int q = 0; q < 4; q++) sF[q * stride + el * C + c] = F[q];
@@ -630,7 +631,8 @@ extern "C" int plbfs_batch_create(const plbfs_mesh_view *mesh, const plbfs_param
         if (params->fneq_min != 0.0) dtcap[c] = case_visc[c] * 3 / params->gamma / params->fneq_min;
     }
     if ((rc = balloc(b, &b->stA, NR * 4 * sb)) || (rc = balloc(b, &b->stB, NR * 4 * sb)) ||
-        (rc = balloc(b, &b->grad, NR * 12 * sb)) || (rc = balloc(b, &b->t1, NR * 4 * sb)) ||
+        (rc = balloc(b, &b->grad, NR * 12 * sb)) ||
+        (rc = balloc(b, &b->t1, NR * 4 * sb)) ||
         (rc = balloc(b, (void **)&b->t0, NR * 4 * sizeof(double))) ||
         (rc = balloc(b, (void **)&b->dtl, NR * sizeof(double))) ||
         (rc = balloc(b, (void **)&b->fbody, NR * 3 * sizeof(double))) ||

@@ -47,6 +47,16 @@ struct V4T<float> {
 template <typename S>
 using V4 = typename V4T<S>::type;
 
+// Gradient records gathered by the face items.  FP64: three V4 (d/dx, d/dy,
+// d/dz of rho, u, v, w), 96 B.  FP32: the cell's state followed by its three
+// gradient V4, 64 B, so that a face side is one aligned 64 B run; the
+// gradient kernel writes all four (the state it read), so its stores stay
+// contiguous.
+template <typename S>
+inline constexpr int kGradRec = std::is_same<S, float>::value ? 4 : 3;  // V4 per record
+template <typename S>
+inline constexpr int kGradOff = std::is_same<S, float>::value ? 1 : 0;  // first gradient V4
+
 // Workgroups are dispatched round-robin over the 8 XCDs (blockIdx.x mod 8), and
 // each XCD has its own L2.  Remapping gives XCD x a contiguous range of logical
 // blocks, so Hilbert-adjacent cells (and their shared halo) meet in one L2.

@@ -87,22 +87,43 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
     const int c0 = xcd_block(blockIdx.x, gridDim.x, m.xcd_remap) * kBlock;
     const int i = c0 + threadIdx.x;
     S g[12];
+    V4<S> si;
+    const int nc = min(m.N - c0, kBlock);
+    const bool live = i < m.N;
     if (m.grad_stage) {
         // the block's own states in LDS: most stencil neighbours are in the block
         __shared__ V4<S> sSt[kBlock];
-        const int nc = min(m.N - c0, kBlock);
         if ((int)threadIdx.x < nc) sSt[threadIdx.x] = st[i];
         __syncthreads();
-        if (i >= m.N) return;
-        lsq_grad<S>(m, i, StagedStates<S>{st, sSt, c0, nc}, g);
-    } else {
-        if (i >= m.N) return;
+        if (live) {
+            lsq_grad<S>(m, i, StagedStates<S>{st, sSt, c0, nc}, g);
+            si = sSt[threadIdx.x];
+        }
+    } else if (live) {
         lsq_grad<S>(m, i, GlobalStates<S, Rec1>{st, Rec1{}}, g);
+        si = st[i];
     }
-    V4<S> *gi = grad + (size_t)3 * i;
-    gi[0] = V4<S>{g[0], g[1], g[2], g[3]};
-    gi[1] = V4<S>{g[4], g[5], g[6], g[7]};
-    gi[2] = V4<S>{g[8], g[9], g[10], g[11]};
+    if constexpr (kGradOff<S> == 1) {
+        // 64 B records (state + gradient): assembled in LDS, then stored by the
+        // block as one contiguous run, lane by lane
+        __shared__ V4<S> sOut[kBlock * kGradRec<S>];
+        if (live) {
+            V4<S> *o = sOut + kGradRec<S> * threadIdx.x;
+            o[0] = si;
+            o[1] = V4<S>{g[0], g[1], g[2], g[3]};
+            o[2] = V4<S>{g[4], g[5], g[6], g[7]};
+            o[3] = V4<S>{g[8], g[9], g[10], g[11]};
+        }
+        __syncthreads();
+        V4<S> *out = grad + (size_t)kGradRec<S> * c0;
+        for (int k = threadIdx.x; k < kGradRec<S> * nc; k += kBlock) out[k] = sOut[k];
+    } else {
+        if (!live) return;
+        V4<S> *gi = grad + (size_t)kGradRec<S> * i;
+        gi[0] = V4<S>{g[0], g[1], g[2], g[3]};
+        gi[1] = V4<S>{g[4], g[5], g[6], g[7]};
+        gi[2] = V4<S>{g[8], g[9], g[10], g[11]};
+    }
 }
 
 // Faces between two flux blocks, one thread per face: flux into the border
@@ -522,7 +543,7 @@ int create_ctx(const plbfs_mesh_view *mesh, const plbfs_params *params, const pl
     const size_t nacc = (size_t)4 * N * (c->f32 ? sizeof(float) : sizeof(double));
     if ((rc = dalloc(c, (void **)&c->stA, n4)) || (rc = dalloc(c, (void **)&c->stB, (size_t)4 * N * sizeof(double))) ||
         (rc = dalloc(c, (void **)&c->t0, (size_t)4 * N * sizeof(double))) || (rc = dalloc(c, (void **)&c->t1, nacc)) ||
-        (rc = dalloc(c, (void **)&c->grad, (size_t)12 * N * sbytes)) ||
+        (rc = dalloc(c, (void **)&c->grad, (size_t)4 * (c->f32 ? kGradRec<float> : kGradRec<double>) * N * sbytes)) ||
         (rc = dalloc(c, (void **)&c->dtl, (size_t)N * sizeof(double))) ||
         (rc = dalloc(c, (void **)&c->fbody, (size_t)3 * N * sizeof(double))) ||
         (rc = dalloc(c, (void **)&c->partials, (size_t)L.n_blocks * 5 * sizeof(double))) ||
@@ -823,16 +844,19 @@ extern "C" int plbfs_eval_residual(plbfs_ctx *c, double *grads, double *R) {
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(c->stream));
     if (grads) {
-        std::vector<double> h2((size_t)12 * N);
+        // records of kGradRec V4 per cell, the gradient from V4 kGradOff on
+        const int rs = 4 * (c->f32 ? kGradRec<float> : kGradRec<double>);
+        const int ro = 4 * (c->f32 ? kGradOff<float> : kGradOff<double>);
+        std::vector<double> h2((size_t)rs * N);
         if (!c->f32) {
             HIP_TRY(hipMemcpy(h2.data(), c->grad, h2.size() * sizeof(double), hipMemcpyDeviceToHost));
         } else {
-            std::vector<float> h1((size_t)12 * N);
+            std::vector<float> h1((size_t)rs * N);
             HIP_TRY(hipMemcpy(h1.data(), c->grad, h1.size() * sizeof(float), hipMemcpyDeviceToHost));
             h2.assign(h1.begin(), h1.end());
         }
         for (int n = 0; n < N; n++)
-            for (int q = 0; q < 12; q++) grads[(size_t)c->L.perm[n] * 12 + q] = h2[(size_t)12 * n + q];
+            for (int q = 0; q < 12; q++) grads[(size_t)c->L.perm[n] * 12 + q] = h2[(size_t)rs * n + ro + q];
     }
     if (R) {
         std::vector<double> h4((size_t)4 * N);

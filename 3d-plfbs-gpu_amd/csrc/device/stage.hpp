@@ -144,7 +144,10 @@ __device__ __forceinline__ void unpack_side(V4<S> q, V4<S> a, V4<S> b, V4<S> d, 
 // the copy the block staged in LDS (sRec: state, grad x3 per cell; sCg:
 // centroid + 1/V), every other cell from global memory.  With nc = 0 every
 // read goes to global memory.
-template <typename S, typename R>
+// COMB: gradient records carry the state (kGradRec<S> V4 per record, the
+// single-case FP32 layout); otherwise the state comes from `st` and the record
+// is the three gradient V4 (FP64, and the batched solver in either precision).
+template <typename S, typename R, bool COMB = (kGradOff<S> == 1)>
 struct SideLoader {
     const V4<S> *__restrict__ st;
     const V4<S> *__restrict__ grad;
@@ -161,8 +164,13 @@ struct SideLoader {
             unpack_side<S>(opaque4(q[0]), opaque4(q[1]), opaque4(q[2]), opaque4(q[3]), p, g);
         } else {
             const size_t r = rec(c);
-            const V4<S> *gc = grad + 3 * r;
-            unpack_side<S>(st[r], gc[0], gc[1], gc[2], p, g);
+            if constexpr (COMB) {
+                const V4<S> *gc = grad + kGradRec<S> * r;
+                unpack_side<S>(gc[0], gc[1], gc[2], gc[3], p, g);
+            } else {
+                const V4<S> *gc = grad + 3 * r;
+                unpack_side<S>(st[r], gc[0], gc[1], gc[2], p, g);
+            }
         }
     }
     __device__ __forceinline__ double4 centroid(int c) const {
@@ -176,7 +184,7 @@ struct SideLoader {
 // lane) for the C cases b0 .. b0 + C - 1 of a batch of B (C = B = 1 for one
 // case; lanes past the last case are left unset), and, when cgeo is given,
 // the cells' centroid records.
-template <typename S>
+template <typename S, bool COMB = (kGradOff<S> == 1)>
 __device__ __forceinline__ void stage_records(const V4<S> *__restrict__ st, const V4<S> *__restrict__ grad,
                                               const double4 *__restrict__ cgeo, int B, int b0, int C, int c0,
                                               int nc, V4<S> *sRec, double4 *sCg) {
@@ -185,7 +193,8 @@ __device__ __forceinline__ void stage_records(const V4<S> *__restrict__ st, cons
         const int il = r / C, cl = r - il * C;
         if (b0 + cl >= B) continue;
         const size_t g = (size_t)(c0 + il) * B + b0 + cl;
-        sRec[k] = q == 0 ? st[g] : grad[3 * g + q - 1];
+        if constexpr (COMB) sRec[k] = grad[kGradRec<S> * g + q];
+        else sRec[k] = q == 0 ? st[g] : grad[3 * g + q - 1];
     }
     if (cgeo)
         for (int k = threadIdx.x; k < nc; k += blockDim.x) sCg[k] = cgeo[c0 + k];
