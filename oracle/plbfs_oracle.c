@@ -52,12 +52,19 @@
  *                      not for 2 (owner value);
  *   ghost gradients    type 2 keeps the sign of n in (1 - n): a field mirror-
  *                      symmetric about a cube's mid-plane gets a residual that
- *                      is symmetric except in the cells on the two Neumann walls.
+ *                      is symmetric except in the cells on the two Neumann walls;
+ *   type 8 (symmetry-x) the ghost is the owner's mirror image (rho, u, w copied,
+ *                      v negated: exact gradients of even/odd linear fields on
+ *                      the plane), and its velocity gradient is the Dirichlet
+ *                      one toward the patch value (a uniform stream is steady
+ *                      when the patch holds it, slowed or sped up in proportion
+ *                      to the difference otherwise).
  * Parity unpinned (restated line by line from the cited source; no vector or
  * closed form distinguishes them from a plausible alternative): the value of
  * the reference LSQ determinant expression on non-axis-aligned stencils, the
  * "neighbour > n_cells" and "face > n_internal" boundary tests, the ghost
- * state of type 8 and the ghost gradients of types 6/7/8.
+ * gradients of types 6 and 7 and the density slot of the type-8 ghost
+ * gradient (left zero).
  */
 #include <math.h>
 #include <stdint.h>

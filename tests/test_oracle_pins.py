@@ -14,7 +14,10 @@ restatement is pinned by what the mathematics fixes:
   RK        - one step of a near-equilibrium single cell equals the reference
               scheme's amplification 1 + Z + 7/12 Z^2 + 5/24 Z^3 + 1/24 Z^4
               (Z = dt J), distinct from classical RK4;
-  forcing   - a uniform body force on a resting uniform state adds dt*f per step.
+  forcing   - a uniform body force on a resting uniform state adds dt*f per step;
+  ghosts    - the free stream is steady in the channel; Neumann ghost gradients
+              keep the sign of n; symmetry-x ghosts mirror the owner and drag
+              the stream toward the patch velocity.
 """
 import numpy as np
 import pytest
@@ -411,3 +414,63 @@ def test_ghost_states_take_the_patch_or_the_owner_value(inlet_vel, inlet_rho):
         x = cells["cx"][: o.N]
         assert np.abs(R_r[x < 0.2 * x.max()]).max() > 1e-6
     assert near_outlet_or_inlet > 1e-6
+
+
+SYM_Y0_PATCHES = {"x0": "wall", "x1": "wall", "y0": "sym", "y1": "wall", "z0": "wall", "z1": "wall"}
+
+
+def test_symmetry_ghost_is_the_mirror_image():
+    # symmetry-x (type 8) ghosts copy rho, u, w and negate v (Solution.cpp:759-833).
+    # On axis-aligned hexes the ghost centroid is the owner's mirror image in the
+    # y = 0 plane, so a field that is even in y for rho, u, w and odd for v is
+    # sampled exactly by the ghosts and the least-squares gradient of the cells
+    # on that plane is exact.  A v that is not odd about the plane is not.
+    n = 6
+    pm = meshgen.box_mesh(n, n, n, patch_types=SYM_Y0_PATCHES)
+    neu = dict(rho=1.0, u=0, v=0, w=0, vel_type="neumann", rho_type="neumann")
+    sym = dict(rho=1.0, u=0, v=0, w=0, vel_type="symmetry-x", rho_type="symmetry-x")
+    case = pc.Case(mach_no=0.05, reynolds_no=100.0, bcs={"wall": neu, "sym": sym})
+    o = oracle_for(pm, case)
+    c = o.cells()
+    N = o.N
+    x, y, z = c["cx"], c["cy"], c["cz"]
+    h = 1.0 / n
+    on_plane = (y[:N] < h) & (x[:N] > h) & (x[:N] < 1 - h) & (z[:N] > h) & (z[:N] < 1 - h)
+    assert on_plane.sum() == (n - 2) ** 2
+    want = np.array([0.1, 0, 0.2, 0.3, 0, -0.1, 0, 0.4, 0, 0.2, 0, 0.05])
+    st = [1 + 0.1 * x + 0.2 * z, 0.3 * x - 0.1 * z, 0.4 * y, 0.2 * x + 0.05 * z]
+    _, g = o.eval_residual(*st)
+    assert np.abs(g[:N][on_plane] - want).max() < 1e-12
+    st[2] = 0.4 * y + 0.1
+    _, g = o.eval_residual(*st)
+    assert np.abs(g[:N][on_plane, 7] - 0.4).min() > 0.1
+
+
+def test_symmetry_patch_velocity_drags_the_stream():
+    # the symmetry-x ghost velocity gradient is the Dirichlet one,
+    # 2 (U_patch - u) / |d| n (gradients.cpp:592-648 via the type-8 branches),
+    # so a uniform stream U along y-normal symmetry sides is steady when the
+    # patch holds U, and the patch value otherwise acts like a sliding wall:
+    # the cells on those sides are slowed by a slower patch, sped up by a
+    # faster one, in proportion to U_patch - U; every other cell stays steady
+    pt = {"x0": "inlet", "x1": "outlet", "y0": "sym", "y1": "sym", "z0": "farfield", "z1": "farfield"}
+    pm = meshgen.box_mesh(10, 5, 5, lengths=(10.0, 5.0, 5.0), tets=True, jitter=0.15, jitter_seed=11,
+                          permute_seed=12, patch_types=pt)
+    ru = {}
+    for u_patch in (1.0, 0.0, 0.5, 2.0):
+        case = pc.channel_case(re=40.0, length_scale=4.0)
+        case.bcs["sym"] = dict(rho=1.0, u=u_patch, v=0.0, w=0.0, vel_type="symmetry-x", rho_type="symmetry-x")
+        case.lsq_exact_det = 1
+        o = oracle_for(pm, case)
+        cells = o.cells()
+        st = case.initial_state(cells["cx"], cells["cy"], cells["cz"])
+        R, _ = o.eval_residual(*st)
+        y = cells["cy"][: o.N]
+        side = (y < 1.0) | (y > 4.0)
+        assert np.abs(R[~side]).max() < 1e-14, u_patch
+        ru[u_patch] = R[side, 1]
+    assert np.abs(ru[1.0]).max() < 1e-14
+    assert ru[0.0].max() < 1e-14 and ru[0.0].min() < -1e-4
+    assert ru[2.0].min() > -1e-14 and ru[2.0].max() > 1e-4
+    # half the velocity difference, half the drag
+    assert abs(ru[0.5].min() / ru[0.0].min() - 0.5) < 0.01
