@@ -1,0 +1,59 @@
+"""GPU parity for the boundary types the other parity tests do not reach:
+symmetry-x (type 8) sides, and an inlet that holds velocity (type 6) and
+density (types 6 and 7).  Same bars as the other FP64 parity tests."""
+import numpy as np
+import pytest
+
+from helpers import oracle_for, perturbed_state, rel_err
+from plbfs import case as pc
+from plbfs import meshgen
+from plbfs.solver import GpuSolver, HostMesh
+
+pytestmark = pytest.mark.gpu
+
+SYM_SIDES = {"x0": "inlet", "x1": "outlet", "y0": "sym", "y1": "sym", "z0": "farfield", "z1": "farfield"}
+
+
+def bc_case(kind):
+    case = pc.channel_case(re=40.0, length_scale=4.0)
+    if kind == "symmetry":
+        case.bcs["sym"] = dict(rho=1.0, u=0.5, v=0.0, w=0.0, vel_type="symmetry-x", rho_type="symmetry-x")
+    elif kind == "inlet6":
+        case.bcs["inlet"] = dict(rho=1.0, u=1.0, v=0.0, w=0.0, vel_type="inlet", rho_type="inlet")
+    else:
+        case.bcs["inlet"] = dict(rho=1.0, u=1.0, v=0.0, w=0.0, vel_type="inlet", rho_type="wall")
+    return case
+
+
+def bc_mesh(kind, tets):
+    pt = SYM_SIDES if kind == "symmetry" else pc.CHANNEL_PATCHES
+    return meshgen.box_mesh(10, 5, 5, lengths=(10.0, 5.0, 5.0), tets=tets, jitter=0.15, jitter_seed=11,
+                            permute_seed=12, patch_types=pt)
+
+
+@pytest.mark.parametrize("kind", ["symmetry", "inlet6", "inlet7"])
+@pytest.mark.parametrize("tets", [False, True])
+def test_boundary_types_match_oracle(gpu_available, kind, tets):
+    pm = bc_mesh(kind, tets)
+    case = bc_case(kind)
+    case.lsq_exact_det = 1 if tets else 0
+    o = oracle_for(pm, case)
+    hm = HostMesh.from_polymesh(pm, case.visc, case.gamma, case.fneq_min)
+    s = GpuSolver(hm, case.params, case.patches_for(pm.patch_names))
+    st = perturbed_state(case, o, amp=0.01, seed=17)
+    print("case", kind, "tets", tets, "cells", pm.n_cells, flush=True)
+    Ro, go = o.eval_residual(*st)
+    s.set_state(*st)
+    R, g = s.eval_residual()
+    N = pm.n_cells
+    for q in range(12):
+        assert rel_err(g[:, q], go[:N, q]) < 1e-11, ("grad", q)
+    for q in range(4):
+        assert rel_err(R[:, q], Ro[:, q]) < 1e-11, ("R", q)
+    o.set_state(*st)
+    s.set_state(*st)
+    n = 60 if tets else 100
+    assert o.run(n)["rc"] == 0
+    assert s.run(n)["rc"] == 0
+    errs = [rel_err(a, b) for a, b in zip(s.state(), o.state())]
+    assert max(errs) < 1e-10, errs
