@@ -3,14 +3,15 @@ symmetry-x (type 8) sides, and an inlet that holds velocity (type 6) and
 density (types 6 and 7).  FP64: the bars of the other FP64 parity tests
 (1e-11 one-stage, 1e-10 after 100 steps, 60 on tets).  FP32 storage: 2e-5
 one-stage residual, 1e-5 velocity and 1e-4 density deviation after the same
-steps."""
+steps.  Batched: each case of a two-viscosity batch equals its single-case
+run bit for bit."""
 import numpy as np
 import pytest
 
 from helpers import oracle_for, perturbed_state, rel_err
 from plbfs import case as pc
 from plbfs import meshgen
-from plbfs.solver import GpuSolver, HostMesh
+from plbfs.solver import GpuBatch, GpuSolver, HostMesh
 
 pytestmark = pytest.mark.gpu
 
@@ -88,3 +89,32 @@ def test_boundary_types_fp32_match_oracle(gpu_available, kind, tets):
     assert rel_err(sg[0] - 1.0, so[0] - 1.0) < 1e-4
     for q in (1, 2, 3):
         assert rel_err(sg[q], so[q]) < 1e-5, q
+
+
+@pytest.mark.parametrize("kind", ["symmetry", "inlet7"])
+@pytest.mark.parametrize("precision", [0, 1])
+def test_batch_boundary_types_equal_single_runs(gpu_available, kind, precision):
+    pm = bc_mesh(kind, False)
+    cases = []
+    for re in (30.0, 90.0):
+        c = bc_case(kind)
+        c.reynolds_no = re
+        c.precision = precision
+        cases.append(c)
+    assert cases[0].visc != cases[1].visc
+    o = oracle_for(pm, cases[0])
+    st = perturbed_state(cases[0], o, amp=0.01, seed=4)
+    print("case", kind, "precision", precision, flush=True)
+    hm = HostMesh.from_polymesh(pm, cases[0].visc, cases[0].gamma, cases[0].fneq_min)
+    bt = GpuBatch(hm, cases[0].params, cases[0].patches_for(pm.patch_names), [c.visc for c in cases])
+    bt.set_state(-1, *st)
+    rc, res = bt.run(40)
+    assert rc == 0 and all(r["steps"] == 40 for r in res), res
+    for b, c in enumerate(cases):
+        h1 = HostMesh.from_polymesh(pm, c.visc, c.gamma, c.fneq_min)
+        s = GpuSolver(h1, c.params, c.patches_for(pm.patch_names))
+        s.set_state(*st)
+        assert s.run(40)["rc"] == 0
+        s1, sb = s.state(), bt.state(b)
+        for q in range(4):
+            assert np.array_equal(sb[q], s1[q]), (b, q, rel_err(sb[q], s1[q]))
