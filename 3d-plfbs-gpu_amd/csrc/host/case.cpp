@@ -11,6 +11,7 @@
 #include <sys/stat.h>
 
 #include <chrono>
+#include <climits>
 #include <fstream>
 #include <cmath>
 #include <cstdio>
@@ -29,6 +30,9 @@
 struct plbfs_mesh;  // defined in mesh.cpp
 
 namespace {
+
+// Upper bound on the nodes of one Lagrangian object read from a case.
+constexpr int kMaxObjectNodes = 100000000;
 
 struct BcSpec {
     std::string name;
@@ -105,14 +109,12 @@ static void flatten(const plbfs::XmlNode &node, const std::string &prefix, plbfs
     }
 }
 
-extern "C" int plbfs_case_load(const char *xml_path, plbfs_case **out) {
-    if (!xml_path || !out) return plbfs_fail(PLBFS_ERR_INPUT, "null argument");
-    *out = nullptr;
+static int case_load(const char *xml_path, plbfs_case **out) {
     std::string text, err;
     if (!read_text(xml_path, text)) return plbfs_fail(PLBFS_ERR_INPUT, "cannot read %s", xml_path);
     auto root = plbfs::xml_parse(text, err);
     if (!root) return plbfs_fail(PLBFS_ERR_INPUT, "%s: %s", xml_path, err.c_str());
-    auto *c = new plbfs_case();
+    std::unique_ptr<plbfs_case> c(new plbfs_case());
     c->xml_path = xml_path;
     flatten(*root, "", *c);
     // required elements (the reference dereferences each without checks)
@@ -121,10 +123,10 @@ extern "C" int plbfs_case_load(const char *xml_path, plbfs_case **out) {
                               "global_variables/mach_no", "global_variables/import_file",
                               "geometry/ref_length", "initial_conditions/average_rho"};
     for (const char *k : required) {
-        if (!c->num.count(k) && !c->str.count(k)) {
-            delete c;
+        if (!c->num.count(k) && !c->str.count(k))
             return plbfs_fail(PLBFS_ERR_INPUT, "%s: missing element %s", xml_path, k);
-        }
+        if (std::strcmp(k, "global_variables/import_file") != 0 && !c->num.count(k))
+            return plbfs_fail(PLBFS_ERR_INPUT, "%s: element %s is not a number", xml_path, k);
     }
     const double U = c->n("global_variables/mach_no");
     const double scale = c->n("global_variables/scale", 1.0);
@@ -157,7 +159,8 @@ extern "C" int plbfs_case_load(const char *xml_path, plbfs_case **out) {
             };
             o.name = t("name");
             o.type = t("type");
-            o.num_nodes = (int)std::atof(t("num_nodes").c_str());
+            const double nn = std::atof(t("num_nodes").c_str());
+            o.num_nodes = nn >= 1 && nn <= kMaxObjectNodes ? (int)nn : -1;
             o.radius = std::atof(t("radius").c_str());
             o.depth = std::atof(t("depth").c_str());
             o.stiffness = std::atof(t("stiffness").c_str());
@@ -193,8 +196,16 @@ extern "C" int plbfs_case_load(const char *xml_path, plbfs_case **out) {
     p.precision = c->s("global_variables/precision") == "fp32" ? PLBFS_FP32 : PLBFS_FP64;
     p.check_interval = (int32_t)c->n("global_variables/check_interval", 10);
     p.lsq_exact_det = c->s("global_variables/lsq_determinant") == "exact" ? 1 : 0;
-    *out = c;
+    if (!std::isfinite(p.visc))
+        return plbfs_fail(PLBFS_ERR_INPUT, "%s: mach_no * ref_length / reynolds_no is not finite", xml_path);
+    *out = c.release();
     return PLBFS_OK;
+}
+
+extern "C" int plbfs_case_load(const char *xml_path, plbfs_case **out) {
+    if (!xml_path || !out) return plbfs_fail(PLBFS_ERR_INPUT, "null argument");
+    *out = nullptr;
+    return plbfs_guarded("plbfs_case_load", [&] { return case_load(xml_path, out); });
 }
 
 extern "C" void plbfs_case_free(plbfs_case *c) { delete c; }
@@ -286,10 +297,11 @@ extern "C" int plbfs_case_initial_state(const plbfs_case *c, const plbfs_mesh *m
 
 extern "C" int plbfs_case_n_objects(const plbfs_case *c) { return c ? (int)c->objects.size() : 0; }
 
-extern "C" int plbfs_case_object_nodes(const plbfs_case *c, int32_t k, double *x, int32_t max_nodes, int32_t *n_out,
-                                       double *stiffness, double *node_area, int32_t *kernel) {
-    if (!c || k < 0 || k >= (int)c->objects.size()) return plbfs_fail(PLBFS_ERR_INPUT, "bad object index");
+static int object_nodes(const plbfs_case *c, int32_t k, double *x, int32_t max_nodes, int32_t *n_out,
+                        double *stiffness, double *node_area, int32_t *kernel) {
     const ObjSpec &o = c->objects[k];
+    if ((o.type == "rigid_cylinder" || o.type == "rigid_sphere") && o.num_nodes < 1)
+        return plbfs_fail(PLBFS_ERR_INPUT, "object '%s': num_nodes must be 1..%d", o.name.c_str(), kMaxObjectNodes);
     const double PI = 3.14159265358979323846;
     std::vector<double> pts;
     double area = 0;
@@ -328,7 +340,8 @@ extern "C" int plbfs_case_object_nodes(const plbfs_case *c, int32_t k, double *x
                                    o.mesh_file.c_str());
         long nn = -1, ns = -1, nt = -1;
         in >> nn >> ns >> nt;
-        if (!in || nn <= 0 || ns < 0 || nt < 0)
+        if (!in || nn <= 0 || ns < 0 || nt < 0 || nn > kMaxObjectNodes || ns > 16L * kMaxObjectNodes ||
+            nt > 16L * kMaxObjectNodes)
             return plbfs_fail(PLBFS_ERR_INPUT, "object '%s': bad counts in '%s'", o.name.c_str(), o.mesh_file.c_str());
         pts.resize((size_t)3 * nn);
         for (long i = 0; i < 3 * nn; i++) in >> pts[i];
@@ -359,6 +372,13 @@ extern "C" int plbfs_case_object_nodes(const plbfs_case *c, int32_t k, double *x
     if (node_area) *node_area = area;
     if (kernel) *kernel = o.kernel;
     return PLBFS_OK;
+}
+
+extern "C" int plbfs_case_object_nodes(const plbfs_case *c, int32_t k, double *x, int32_t max_nodes, int32_t *n_out,
+                                       double *stiffness, double *node_area, int32_t *kernel) {
+    if (!c || k < 0 || k >= (int)c->objects.size()) return plbfs_fail(PLBFS_ERR_INPUT, "bad object index");
+    return plbfs_guarded("plbfs_case_object_nodes",
+                         [&] { return object_nodes(c, k, x, max_nodes, n_out, stiffness, node_area, kernel); });
 }
 
 // ---- output ----------------------------------------------------------------
@@ -446,7 +466,7 @@ extern "C" int plbfs_write_tecplot(const plbfs_mesh *m, const char *path, const 
 
 // ---- program ---------------------------------------------------------------
 
-extern "C" int plbfs_program_run(const char *xml_path, plbfs_result *result) {
+static int program_run(const char *xml_path, plbfs_result *result) {
     const auto t_start = std::chrono::steady_clock::now();
     plbfs_case *c = nullptr;
     int rc = plbfs_case_load(xml_path, &c);
@@ -506,7 +526,10 @@ extern "C" int plbfs_program_run(const char *xml_path, plbfs_result *result) {
         nodes.kernel = kernel;
         if ((rc = plbfs_set_ibm(ctx, &nodes))) return rc;
     }
-    const int steps = (int)std::ceil(c->n("global_variables/simulation_length") * c->n("global_variables/scale", 1.0));
+    const double length = std::ceil(c->n("global_variables/simulation_length") * c->n("global_variables/scale", 1.0));
+    if (!(length >= 0 && length <= INT32_MAX))
+        return plbfs_fail(PLBFS_ERR_INPUT, "simulation_length * scale = %g steps is out of range", length);
+    const int steps = (int)length;
     plbfs_result r{};
     rc = plbfs_run(ctx, steps, &r);
     if (rc != PLBFS_OK && rc != PLBFS_ERR_NAN && rc != PLBFS_ERR_RHO && rc != PLBFS_ERR_IBM) return rc;
@@ -569,4 +592,8 @@ extern "C" int plbfs_program_run(const char *xml_path, plbfs_result *result) {
     }
     if (result) *result = r;
     return run_rc;
+}
+
+extern "C" int plbfs_program_run(const char *xml_path, plbfs_result *result) {
+    return plbfs_guarded("plbfs_program_run", [&] { return program_run(xml_path, result); });
 }

@@ -16,7 +16,9 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <climits>
 #include <fstream>
+#include <memory>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -187,7 +189,10 @@ static int validate_topology(const plbfs_mesh &m) {
             return plbfs_fail(PLBFS_ERR_INPUT, "face %d neighbour", f);
     }
     long total = 0;
-    for (int32_t s : m.patch_sizes) total += s;
+    for (int32_t s : m.patch_sizes) {
+        if (s < 0) return plbfs_fail(PLBFS_ERR_INPUT, "negative patch size %d", s);
+        total += s;
+    }
     if (total != m.n_faces - m.n_internal)
         return plbfs_fail(PLBFS_ERR_INPUT, "patch sizes sum %ld != boundary faces %d", total,
                           m.n_faces - m.n_internal);
@@ -200,16 +205,10 @@ static void fill_bface_patch(plbfs_mesh &m) {
         for (int q = 0; q < m.patch_sizes[p]; q++) m.bface_patch.push_back((int32_t)p);
 }
 
-extern "C" int plbfs_mesh_read_openfoam(const char *prefix, double visc, double gamma, double fneq_min,
-                                        plbfs_mesh **out) {
-    if (!prefix || !out) return plbfs_fail(PLBFS_ERR_INPUT, "null argument");
-    *out = nullptr;
+static int read_openfoam(const char *prefix, double visc, double gamma, double fneq_min, plbfs_mesh **out) {
     std::string pre(prefix), text;
-    auto *m = new plbfs_mesh();
-    auto bail = [&](int rc) {
-        delete m;
-        return rc;
-    };
+    std::unique_ptr<plbfs_mesh> m(new plbfs_mesh());
+    auto bail = [](int rc) { return rc; };
     // owner: counts from the note on 0-based line 12
     if (!read_file(pre + "owner", text)) return bail(plbfs_fail(PLBFS_ERR_INPUT, "cannot read %sowner", prefix));
     {
@@ -225,6 +224,11 @@ extern "C" int plbfs_mesh_read_openfoam(const char *prefix, double visc, double 
         if (!parse_note(line, "nCells:", nc) || !parse_note(line, "nFaces:", nf) ||
             !parse_note(line, "nInternalFaces:", ni) || !parse_note(line, "nPoints:", np))
             return bail(plbfs_fail(PLBFS_ERR_INPUT, "owner line 12 lacks the nPoints/nCells/nFaces note"));
+        // every owner entry takes at least two bytes of the file
+        if (nc <= 0 || nf <= 0 || np <= 0 || ni < 0 || ni > nf || nc > INT32_MAX || np > INT32_MAX ||
+            nf > (long)(text.size() / 2))
+            return bail(plbfs_fail(PLBFS_ERR_INPUT, "owner note counts out of range: nPoints %ld nCells %ld nFaces %ld "
+                                   "nInternalFaces %ld", np, nc, nf, ni));
         m->n_cells = (int32_t)nc;
         m->n_faces = (int32_t)nf;
         m->n_internal = (int32_t)ni;
@@ -239,6 +243,9 @@ extern "C" int plbfs_mesh_read_openfoam(const char *prefix, double visc, double 
         if (i != m->n_faces) return bail(plbfs_fail(PLBFS_ERR_INPUT, "owner has %d entries, expected %d", i, m->n_faces));
     }
     if (!read_file(pre + "points", text)) return bail(plbfs_fail(PLBFS_ERR_INPUT, "cannot read %spoints", prefix));
+    // a point line "(x y z)" takes at least eight bytes
+    if ((size_t)m->n_points > text.size() / 8)
+        return bail(plbfs_fail(PLBFS_ERR_INPUT, "points file too short for %d points", m->n_points));
     m->px.assign(m->n_points, 0.f);
     m->py.assign(m->n_points, 0.f);
     m->pz.assign(m->n_points, 0.f);
@@ -303,6 +310,7 @@ extern "C" int plbfs_mesh_read_openfoam(const char *prefix, double visc, double 
     if (!read_file(pre + "boundary", text)) return bail(plbfs_fail(PLBFS_ERR_INPUT, "cannot read %sboundary", prefix));
     {
         std::string bc_type;
+        bool bad = false;
         for_list_lines(text, [&](long, const char *b, const char *e) {
             std::istringstream iss(std::string(b, e));
             std::string t1, t2;
@@ -313,17 +321,66 @@ extern "C" int plbfs_mesh_read_openfoam(const char *prefix, double visc, double 
             }
             if (t1 == "nFaces" && !t2.empty()) {
                 t2.pop_back();
+                char *end = nullptr;
+                errno = 0;
+                long nfp = std::strtol(t2.c_str(), &end, 10);
+                if (end == t2.c_str() || *end != '\0' || errno != 0 || nfp < 0 || nfp > INT32_MAX) {
+                    bad = true;
+                    return;
+                }
                 m->patch_types.push_back(bc_type);
-                m->patch_sizes.push_back((int32_t)std::stol(t2));
+                m->patch_sizes.push_back((int32_t)nfp);
             }
         });
+        if (bad) return bail(plbfs_fail(PLBFS_ERR_INPUT, "boundary: nFaces is not a face count"));
     }
-    fill_bface_patch(*m);
     int rc = validate_topology(*m);
     if (rc != PLBFS_OK) return bail(rc);
+    fill_bface_patch(*m);
     rc = build_geometry(*m, visc, gamma, fneq_min);
     if (rc != PLBFS_OK) return bail(rc);
-    *out = m;
+    *out = m.release();
+    return PLBFS_OK;
+}
+
+extern "C" int plbfs_mesh_read_openfoam(const char *prefix, double visc, double gamma, double fneq_min,
+                                        plbfs_mesh **out) {
+    if (!prefix || !out) return plbfs_fail(PLBFS_ERR_INPUT, "null argument");
+    *out = nullptr;
+    return plbfs_guarded("plbfs_mesh_read_openfoam", [&] { return read_openfoam(prefix, visc, gamma, fneq_min, out); });
+}
+
+static int from_arrays(int32_t n_points, const float *px, const float *py, const float *pz, int32_t n_cells,
+                       int32_t n_faces, int32_t n_internal, const int32_t *face_off, const int32_t *face_verts,
+                       const int32_t *owner, const int32_t *neighbour, int32_t n_patches,
+                       const char *const *patch_types, const int32_t *patch_sizes, double visc, double gamma,
+                       double fneq_min, plbfs_mesh **out) {
+    std::unique_ptr<plbfs_mesh> m(new plbfs_mesh());
+    m->n_points = n_points;
+    m->n_cells = n_cells;
+    m->n_faces = n_faces;
+    m->n_internal = n_internal;
+    m->px.assign(px, px + n_points);
+    m->py.assign(py, py + n_points);
+    m->pz.assign(pz, pz + n_points);
+    m->face_off.assign(face_off, face_off + n_faces + 1);
+    for (int f = 0; f < n_faces; f++)
+        if (face_off[f + 1] < face_off[f]) return plbfs_fail(PLBFS_ERR_INPUT, "face_off decreases at face %d", f);
+    if (m->face_off[0] != 0) return plbfs_fail(PLBFS_ERR_INPUT, "face_off[0] must be 0");
+    m->face_verts.assign(face_verts, face_verts + face_off[n_faces]);
+    m->owner.assign(owner, owner + n_faces);
+    m->neighbour.assign(n_faces, 0);
+    for (int f = 0; f < n_internal; f++) m->neighbour[f] = neighbour[f];
+    for (int p = 0; p < n_patches; p++) {
+        m->patch_types.push_back(patch_types && patch_types[p] ? patch_types[p] : "");
+        m->patch_sizes.push_back(patch_sizes[p]);
+    }
+    int rc = validate_topology(*m);
+    if (rc != PLBFS_OK) return rc;
+    fill_bface_patch(*m);
+    rc = build_geometry(*m, visc, gamma, fneq_min);
+    if (rc != PLBFS_OK) return rc;
+    *out = m.release();
     return PLBFS_OK;
 }
 
@@ -334,38 +391,14 @@ extern "C" int plbfs_mesh_from_arrays(int32_t n_points, const float *px, const f
                                       const int32_t *patch_sizes, double visc, double gamma, double fneq_min,
                                       plbfs_mesh **out) {
     if (!out || !px || !py || !pz || !face_off || !face_verts || !owner || (n_internal > 0 && !neighbour) ||
-        n_points <= 0 || n_faces <= 0 || n_patches < 0)
+        (n_patches > 0 && !patch_sizes) || n_points <= 0 || n_cells <= 0 || n_faces <= 0 || n_internal < 0 ||
+        n_internal > n_faces || n_patches < 0)
         return plbfs_fail(PLBFS_ERR_INPUT, "bad arguments to plbfs_mesh_from_arrays");
-    auto *m = new plbfs_mesh();
-    m->n_points = n_points;
-    m->n_cells = n_cells;
-    m->n_faces = n_faces;
-    m->n_internal = n_internal;
-    m->px.assign(px, px + n_points);
-    m->py.assign(py, py + n_points);
-    m->pz.assign(pz, pz + n_points);
-    m->face_off.assign(face_off, face_off + n_faces + 1);
-    if (m->face_off[0] != 0) {
-        delete m;
-        return plbfs_fail(PLBFS_ERR_INPUT, "face_off[0] must be 0");
-    }
-    m->face_verts.assign(face_verts, face_verts + face_off[n_faces]);
-    m->owner.assign(owner, owner + n_faces);
-    m->neighbour.assign(n_faces, 0);
-    for (int f = 0; f < n_internal; f++) m->neighbour[f] = neighbour[f];
-    for (int p = 0; p < n_patches; p++) {
-        m->patch_types.push_back(patch_types && patch_types[p] ? patch_types[p] : "");
-        m->patch_sizes.push_back(patch_sizes[p]);
-    }
-    fill_bface_patch(*m);
-    int rc = validate_topology(*m);
-    if (rc == PLBFS_OK) rc = build_geometry(*m, visc, gamma, fneq_min);
-    if (rc != PLBFS_OK) {
-        delete m;
-        return rc;
-    }
-    *out = m;
-    return PLBFS_OK;
+    *out = nullptr;
+    return plbfs_guarded("plbfs_mesh_from_arrays", [&] {
+        return from_arrays(n_points, px, py, pz, n_cells, n_faces, n_internal, face_off, face_verts, owner, neighbour,
+                           n_patches, patch_types, patch_sizes, visc, gamma, fneq_min, out);
+    });
 }
 
 extern "C" void plbfs_mesh_free(plbfs_mesh *m) { delete m; }

@@ -7,6 +7,10 @@ namespace plbfs {
 
 namespace {
 
+// Case files nest a handful of levels; deeper documents are rejected rather
+// than recursed into without bound.
+constexpr int kMaxDepth = 64;
+
 struct Parser {
     const std::string &s;
     size_t i = 0;
@@ -70,9 +74,13 @@ struct Parser {
         while (b > a && std::isspace((unsigned char)t[b - 1])) b--;
         return t.substr(a, b - a);
     }
-    std::unique_ptr<XmlNode> element() {
+    std::unique_ptr<XmlNode> element(int depth = 0) {
         if (eof() || s[i] != '<') {
             fail("expected element");
+            return nullptr;
+        }
+        if (depth >= kMaxDepth) {
+            fail("elements nested deeper than " + std::to_string(kMaxDepth));
             return nullptr;
         }
         i++;
@@ -136,7 +144,7 @@ struct Parser {
                 continue;
             }
             if (s[i] == '<') {
-                auto c = element();
+                auto c = element(depth + 1);
                 if (!c) return nullptr;
                 node->children.push_back(std::move(c));
                 continue;

@@ -87,6 +87,33 @@ def test_reader_errors_are_clean(tmp_path):
     assert e.value.rc == -1
 
 
+@pytest.mark.parametrize("fname,pattern,repl", [
+    ("owner", r"nFaces:\d+", "nFaces:-5"),
+    ("owner", r"nFaces:\d+", "nFaces:99999999999"),
+    ("owner", r"nPoints:\d+", "nPoints:2000000000"),
+    ("owner", r"nInternalFaces:\d+", "nInternalFaces:100000"),
+    ("boundary", r"nFaces\s+\d+;", "nFaces abc;"),
+    ("boundary", r"nFaces\s+\d+;", "nFaces 99999999999999999999;"),
+])
+def test_reader_rejects_bad_counts(tmp_path, fname, pattern, repl):
+    # counts that are negative, overflow int32, exceed what the file can hold
+    # or are not numbers are input errors, not crashes or giant allocations
+    import shutil
+    pm = cavity_mesh(3)
+    good = meshgen.write_polymesh(pm, str(tmp_path / "good"))
+    HostMesh.read_openfoam(good, visc=0.001)
+    bad = str(tmp_path / "bad")
+    shutil.copytree(good, bad)
+    path = os.path.join(bad, fname)
+    text = open(path).read()
+    new, n = re.subn(pattern, repl, text, count=1)
+    assert n == 1
+    open(path, "w").write(new)
+    with pytest.raises(nat.PlbfsError) as e:
+        HostMesh.read_openfoam(bad + "/", visc=0.001)
+    assert e.value.rc == -1
+
+
 @pytest.mark.parametrize("tets,jitter", [(False, 0.15), (True, 0.1)])
 def test_product_geometry_equals_oracle_bitwise(tets, jitter):
     pm = cavity_mesh(5, tets=tets, jitter=jitter)
@@ -192,6 +219,41 @@ def test_unknown_patch_token_is_a_clean_error(tmp_path):
     hm = HostMesh.read_openfoam(prefix, 0.001)
     patches = (nat.Patch * hm.n_patches)()
     assert L.plbfs_case_patches(h, hm.h, patches, hm.n_patches) == -4   # 'wall'/'lid' have no <bc>
+    L.plbfs_case_free(h)
+
+
+@pytest.mark.parametrize("edit", ["re_text", "re_zero", "deep", "sphere_nodes"])
+def test_bad_case_values_are_clean_errors(tmp_path, edit):
+    pm = cavity_mesh(3)
+    prefix = meshgen.write_polymesh(pm, str(tmp_path / "mesh"))
+    text = CASE_XML.format(out=str(tmp_path / "out"), mesh=prefix, prec="false")
+    if edit == "re_text":
+        text = text.replace("<reynolds_no>40</reynolds_no>", "<reynolds_no>forty</reynolds_no>")
+    elif edit == "re_zero":
+        text = text.replace("<reynolds_no>40</reynolds_no>", "<reynolds_no>0</reynolds_no>")
+    elif edit == "deep":
+        text = text.replace("</root>", "<x>" * 100 + "</x>" * 100 + "</root>")
+    else:
+        text = text.replace("</root>", "<lagrangian_objects><object><name>s</name><num_nodes>1e20</num_nodes>"
+                            "<radius>1</radius><depth>1</depth><stiffness>1</stiffness><centre_x>0</centre_x>"
+                            "<centre_y>0</centre_y><centre_z>0</centre_z><type>rigid_sphere</type></object>"
+                            "</lagrangian_objects></root>")
+    xml = tmp_path / "case.xml"
+    xml.write_text(text)
+    L = nat.lib()
+    L.plbfs_case_load.argtypes = [C.c_char_p, C.POINTER(C.c_void_p)]
+    h = C.c_void_p()
+    rc = L.plbfs_case_load(str(xml).encode(), C.byref(h))
+    if edit != "sphere_nodes":
+        assert rc == -1 and not h.value
+        return
+    assert rc == 0
+    L.plbfs_case_object_nodes.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_double), C.c_int32,
+                                          C.POINTER(C.c_int32), C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                          C.POINTER(C.c_int32)]
+    n = C.c_int32(0)
+    assert L.plbfs_case_object_nodes(h, 0, None, 0, C.byref(n), None, None, None) == -1
+    L.plbfs_case_free.argtypes = [C.c_void_p]
     L.plbfs_case_free(h)
 
 
