@@ -395,3 +395,13 @@ def test_warm_start_training_schedule_and_field_filter():
     assert ws.usable(ok)
     assert not ws.usable(dict(ok, status=-3))
     assert not ws.usable(dict(ok, residual=[0, 5e-3, 0, 0]))
+
+
+def test_bench_refuses_more_than_one_gpu():
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU only" in r.stderr
+    assert r.stdout == ""
