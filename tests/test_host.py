@@ -405,3 +405,25 @@ def test_bench_refuses_more_than_one_gpu():
     assert r.returncode == 2
     assert "one GPU only" in r.stderr
     assert r.stdout == ""
+
+
+def test_program_reads_the_case_up_to_the_device(tmp_path):
+    # the plbfs executable (src/main.cpp entry) runs the XML case through the
+    # reader, the patch table and the initial conditions; without a device it
+    # stops at plbfs_create with PLBFS_ERR_NODEVICE, and a bad case stops earlier
+    if nat.lib().plbfs_device_count() > 0:
+        pytest.skip("a device is visible")
+    import subprocess
+    exe = os.path.join(ROOT, "3d-plfbs-gpu_amd", "build", "plbfs")
+    pm = channel_mesh(10, 5, 5, jitter=0.1)
+    prefix = meshgen.write_polymesh(pm, str(tmp_path / "mesh"))
+    xml = tmp_path / "case.xml"
+    xml.write_text(CASE_XML.format(out=str(tmp_path / "out"), mesh=prefix, prec="false"))
+    r = subprocess.run([exe, str(xml)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "error -6" in r.stderr, r.stderr
+    bad = tmp_path / "bad.xml"
+    bad.write_text(CASE_XML.format(out=str(tmp_path / "out"), mesh=str(tmp_path / "nomesh") + "/", prec="false"))
+    r = subprocess.run([exe, str(bad)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "error -1" in r.stderr and "owner" in r.stderr, r.stderr
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "usage" in r.stderr
