@@ -112,6 +112,27 @@ def make_patches(patches):
     return arr
 
 
+def state_arrays(arrs, n):
+    """The first n values of rho, u, v, w as contiguous float64 arrays.  The
+    C-ABI reads n values from each pointer, so a shorter input is an error."""
+    out = []
+    for name, x in zip(("rho", "u", "v", "w"), arrs):
+        a = np.asarray(x, dtype=np.float64).reshape(-1)
+        if a.size < n:
+            raise ValueError("%s has %d values; the mesh has %d cells" % (name, a.size, n))
+        out.append(np.ascontiguousarray(a[:n]))
+    return out
+
+
+def node_arrays(x_ref, x):
+    """Anchor and current node positions as flat float64 [3n] arrays of one n."""
+    xr = np.ascontiguousarray(x_ref, dtype=np.float64).reshape(-1)
+    xx = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
+    if xr.size % 3 or xx.size != xr.size:
+        raise ValueError("node arrays must both hold 3n values (got %d and %d)" % (xr.size, xx.size))
+    return xr, xx
+
+
 class GpuSolver:
     """One solver context on the current GPU."""
 
@@ -139,7 +160,7 @@ class GpuSolver:
             pass
 
     def set_state(self, rho, u, v, w):
-        a = [np.ascontiguousarray(x[: self.N], dtype=np.float64) for x in (rho, u, v, w)]
+        a = state_arrays((rho, u, v, w), self.N)
         nat.check(nat.lib().plbfs_set_state(self.h, *(nat.dptr(x) for x in a)))
 
     def state(self):
@@ -169,8 +190,7 @@ class GpuSolver:
         return d
 
     def set_ibm(self, x_ref, x, stiffness, node_area, dt, h, kernel):
-        xr = np.ascontiguousarray(x_ref, dtype=np.float64).reshape(-1)
-        xx = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
+        xr, xx = node_arrays(x_ref, x)
         self._ibm_keep = (xr, xx)
         nodes = nat.IbmNodes(xr.size // 3, nat.dptr(xr), nat.dptr(xx), stiffness, node_area, dt, h, kernel)
         nat.check(nat.lib().plbfs_set_ibm(self.h, C.byref(nodes)))
@@ -247,7 +267,7 @@ class GpuBatch:
             pass
 
     def set_state(self, case, rho, u, v, w):
-        a = [np.ascontiguousarray(x[: self.N], dtype=np.float64) for x in (rho, u, v, w)]
+        a = state_arrays((rho, u, v, w), self.N)
         nat.check(nat.lib().plbfs_batch_set_state(self.h, int(case), *(nat.dptr(x) for x in a)))
 
     def state(self, case):
@@ -256,8 +276,7 @@ class GpuBatch:
         return a
 
     def set_ibm(self, case, x_ref, x, stiffness, node_area, dt, h, kernel):
-        xr = np.ascontiguousarray(x_ref, dtype=np.float64).reshape(-1)
-        xx = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
+        xr, xx = node_arrays(x_ref, x)
         self._ibm_keep[case] = (xr, xx)
         nodes = nat.IbmNodes(xr.size // 3, nat.dptr(xr), nat.dptr(xx), stiffness, node_area, dt, h, kernel)
         nat.check(nat.lib().plbfs_batch_set_ibm(self.h, int(case), C.byref(nodes)))

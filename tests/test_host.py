@@ -427,3 +427,19 @@ def test_program_reads_the_case_up_to_the_device(tmp_path):
     assert r.returncode == 1 and "error -1" in r.stderr and "owner" in r.stderr, r.stderr
     r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
     assert r.returncode == 1 and "usage" in r.stderr
+
+
+def test_wrapper_rejects_short_state_and_node_arrays():
+    # the C-ABI reads n values per pointer: the Python wrappers refuse shorter inputs
+    from plbfs.solver import node_arrays, state_arrays
+    full = [np.arange(12.0)] * 4
+    out = state_arrays(full, 10)
+    assert all(a.shape == (10,) and a.flags.c_contiguous for a in out)
+    with pytest.raises(ValueError):
+        state_arrays([np.arange(12.0), np.arange(9.0), np.arange(12.0), np.arange(12.0)], 10)
+    xr, xx = node_arrays(np.zeros((4, 3)), np.ones((4, 3)))
+    assert xr.shape == xx.shape == (12,)
+    with pytest.raises(ValueError):
+        node_arrays(np.zeros((4, 3)), np.zeros((3, 3)))
+    with pytest.raises(ValueError):
+        node_arrays(np.zeros(10), np.zeros(10))
