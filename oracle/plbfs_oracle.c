@@ -58,12 +58,15 @@
  *                      the plane), and its velocity gradient is the Dirichlet
  *                      one toward the patch value (a uniform stream is steady
  *                      when the patch holds it, slowed or sped up in proportion
- *                      to the difference otherwise).
+ *                      to the difference otherwise);
+ *   type 7 density     the ghost gradient is the owner's: with the patch value
+ *                      on a linear field, the boundary flux equals the flux of
+ *                      the same face inside a longer box (types 1, 2, 6 do not).
  * Parity unpinned (restated line by line from the cited source; no vector or
  * closed form distinguishes them from a plausible alternative): the value of
  * the reference LSQ determinant expression on non-axis-aligned stencils, the
  * "neighbour > n_cells" and "face > n_internal" boundary tests, the ghost
- * gradients of types 6 and 7 and the density slot of the type-8 ghost
+ * gradients of type 6 (zero) and the density slot of the type-8 ghost
  * gradient (left zero).
  */
 #include <math.h>

@@ -17,7 +17,8 @@ restatement is pinned by what the mathematics fixes:
   forcing   - a uniform body force on a resting uniform state adds dt*f per step;
   ghosts    - the free stream is steady in the channel; Neumann ghost gradients
               keep the sign of n; symmetry-x ghosts mirror the owner and drag
-              the stream toward the patch velocity.
+              the stream toward the patch velocity; a type-7 density ghost
+              continues the owner's gradient.
 """
 import numpy as np
 import pytest
@@ -474,3 +475,50 @@ def test_symmetry_patch_velocity_drags_the_stream():
     assert ru[2.0].min() > -1e-14 and ru[2.0].max() > 1e-4
     # half the velocity difference, half the drag
     assert abs(ru[0.5].min() / ru[0.0].min() - 0.5) < 0.01
+
+
+def _x0_face_fluxes(rho_type_x0):
+    # rho = a + b x with a uniform stream U x, on a 4-cell-long box whose x = 0
+    # patch has the given density type (patch value = the field at the ghost
+    # centroid x = -1/2, velocity Dirichlet U), and on an 8-cell box over
+    # [-4, 4] where x = 0 is interior.  The sides are Neumann and the far ends
+    # Dirichlet at the field's face value, so every ghost outside x = 0
+    # reproduces the linear field and every LSQ gradient near x = 0 is exact.
+    # Returns the fluxes of the x = 0 faces of both meshes, matched by
+    # centroid and given the box's outward orientation.
+    a, b = 1.0, 0.02
+    pt = {"x0": "inl", "x1": "out", "y0": "side", "y1": "side", "z0": "side", "z1": "side"}
+    side = dict(rho=1.0, u=0.0, v=0.0, w=0.0, vel_type="neumann", rho_type="neumann")
+    out = dict(rho=a + 4 * b, u=1.0, v=0.0, w=0.0, vel_type="neumann", rho_type="dirichlet")
+    meshes = [(meshgen.box_mesh(4, 3, 3, lengths=(4.0, 3.0, 3.0), patch_types=pt),
+               {"inl": dict(rho=a - 0.5 * b, u=1.0, v=0.0, w=0.0, vel_type="dirichlet", rho_type=rho_type_x0),
+                "out": out, "side": side}),
+              (meshgen.box_mesh(8, 3, 3, lengths=(8.0, 3.0, 3.0), origin=(-4.0, 0.0, 0.0), patch_types=pt),
+               {"inl": dict(out, rho=a - 4 * b), "out": out, "side": side})]
+    fluxes = []
+    for pm, bcs in meshes:
+        case = pc.Case(mach_no=0.05, reynolds_no=40.0, ref_length=4.0, bcs=bcs)
+        case.lsq_exact_det = 1
+        o = oracle_for(pm, case)
+        x = o.cells()["cx"]
+        o.eval_residual(a + b * x, np.full(o.T, case.mach_no), np.zeros(o.T), np.zeros(o.T))
+        f = o.faces()
+        sel = np.where(np.abs(f["fx"]) < 1e-9)[0]
+        assert len(sel) == 9
+        order = np.lexsort((f["fz"][sel], f["fy"][sel]))
+        fluxes.append(np.array([o.face_flux(int(k)) * -np.sign(f["ni"][k]) for k in sel[order]]))
+    return fluxes
+
+
+def test_wall_density_ghost_continues_the_owner_gradient():
+    # density type 7 ("wall") holds the patch value in the ghost and copies the
+    # owner's gradient to it (gradients.cpp:395-504): with the patch value on
+    # the linear field, its boundary flux equals the flux of the same face
+    # inside a longer box.  Type 6 ("inlet", same ghost value, zero ghost
+    # gradient) does not, nor do 1 and 2.
+    fa, fb = _x0_face_fluxes("wall")
+    scale = np.abs(fb).max()
+    assert np.abs(fa - fb).max() < 1e-13 * scale
+    for other in ("inlet", "neumann", "dirichlet"):
+        fa, fb = _x0_face_fluxes(other)
+        assert np.abs(fa - fb).max() > 1e-3 * scale, other
