@@ -443,3 +443,49 @@ def test_wrapper_rejects_short_state_and_node_arrays():
         node_arrays(np.zeros((4, 3)), np.zeros((3, 3)))
     with pytest.raises(ValueError):
         node_arrays(np.zeros(10), np.zeros(10))
+
+
+C_CONSUMER = r"""
+#include <stdio.h>
+#include "plbfs_hip.h"
+#include "plbfs_case.h"
+#include "plbfs_mpnn.h"
+int main(int argc, char **argv) {
+    plbfs_mesh *m = NULL;
+    int64_t info[6];
+    plbfs_mesh_view v;
+    if (argc < 2) return 2;
+    if (plbfs_mesh_read_openfoam(argv[1], 0.001, 1.0, 0.0, &m) != PLBFS_OK) {
+        printf("error %s\n", plbfs_last_error());
+        return 1;
+    }
+    plbfs_mesh_info(m, info);
+    plbfs_mesh_view_get(m, &v);
+    printf("%lld %lld %d %d\n", (long long)info[0], (long long)v.n_faces, plbfs_mpnn_n_params(),
+           plbfs_device_count() >= 0);
+    plbfs_mesh_free(m);
+    return 0;
+}
+"""
+
+
+def test_headers_are_plain_c(tmp_path):
+    # what a cgo / ctypes / JNI binding sees: the three headers compile as
+    # pedantic C99 without warnings, and a C program links the library and
+    # reads a mesh through it
+    import subprocess
+    src = tmp_path / "consumer.c"
+    src.write_text(C_CONSUMER)
+    exe = tmp_path / "consumer"
+    build = os.path.join(ROOT, "3d-plfbs-gpu_amd", "build")
+    r = subprocess.run(["gcc", "-std=c99", "-Wall", "-Wextra", "-Werror", "-pedantic",
+                        "-I" + os.path.join(ROOT, "include"), str(src), "-L" + build, "-lplbfs",
+                        "-Wl,-rpath," + build, "-o", str(exe)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    pm = cavity_mesh(3)
+    prefix = meshgen.write_polymesh(pm, str(tmp_path / "mesh"))
+    r = subprocess.run([str(exe), prefix], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    cells, faces, n_params, ok = r.stdout.split()
+    assert (int(cells), int(faces), int(ok)) == (pm.n_cells, pm.n_faces, 1)
+    assert int(n_params) > 0
