@@ -61,13 +61,15 @@
  *                      to the difference otherwise);
  *   type 7 density     the ghost gradient is the owner's: with the patch value
  *                      on a linear field, the boundary flux equals the flux of
- *                      the same face inside a longer box (types 1, 2, 6 do not).
+ *                      the same face inside a longer box (types 1, 2, 6 do not);
+ *   empty patches      "face > n_internal": on a one-cell-thick box with empty x
+ *                      sides every residual equals the periodic-x one except
+ *                      the owner of boundary face 0, which keeps that face's flux.
  * Parity unpinned (restated line by line from the cited source; no vector or
  * closed form distinguishes them from a plausible alternative): the value of
  * the reference LSQ determinant expression on non-axis-aligned stencils, the
- * "neighbour > n_cells" and "face > n_internal" boundary tests, the ghost
- * gradients of type 6 (zero) and the density slot of the type-8 ghost
- * gradient (left zero).
+ * "neighbour > n_cells" boundary test, the ghost gradients of type 6 (zero)
+ * and the density slot of the type-8 ghost gradient (left zero).
  */
 #include <math.h>
 #include <stdint.h>

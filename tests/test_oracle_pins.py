@@ -18,7 +18,8 @@ restatement is pinned by what the mathematics fixes:
   ghosts    - the free stream is steady in the channel; Neumann ghost gradients
               keep the sign of n; symmetry-x ghosts mirror the owner and drag
               the stream toward the patch velocity; a type-7 density ghost
-              continues the owner's gradient.
+              continues the owner's gradient; empty patches are skipped from
+              the second boundary face on.
 """
 import numpy as np
 import pytest
@@ -522,3 +523,41 @@ def test_wall_density_ghost_continues_the_owner_gradient():
     for other in ("inlet", "neumann", "dirichlet"):
         fa, fb = _x0_face_fluxes(other)
         assert np.abs(fa - fb).max() > 1e-3 * scale, other
+
+
+def _thin_box_residual(x_sides):
+    # one cell thick in x, a field of y and z only: the x faces carry fluxes
+    # that cancel pairwise when both are computed (periodic x, where each
+    # cell is its own partner) and are skipped on "empty" patches
+    if x_sides == "empty":
+        pt_x = {"x0": "empty", "x1": "empty"}
+        bcs = {"empty": dict(rho=1.0, u=0.0, v=0.0, w=0.0, vel_type="neumann", rho_type="neumann")}
+    else:
+        pt_x = {"x0": "xlo", "x1": "xhi"}
+        per = dict(rho=1.0, u=0.0, v=0.0, w=0.0, vel_type="periodic", rho_type="periodic")
+        bcs = {"xlo": dict(per, per_translate=(1.0, 0.0, 0.0)), "xhi": dict(per, per_translate=(-1.0, 0.0, 0.0))}
+    bcs["wall"] = dict(rho=1.0, u=0.0, v=0.0, w=0.0, vel_type="dirichlet", rho_type="neumann")
+    pm = meshgen.box_mesh(1, 5, 4, lengths=(1.0, 5.0, 4.0), permute_seed=3,
+                          patch_types=dict(pt_x, y0="wall", y1="wall", z0="wall", z1="wall"))
+    o = oracle_for(pm, pc.Case(mach_no=0.05, reynolds_no=40.0, ref_length=4.0, bcs=bcs))
+    c = o.cells()
+    y, z = c["cy"], c["cz"]
+    st = [1.0 + 0.01 * np.sin(y) * np.cos(z), 0.05 * np.cos(0.7 * y + 0.3 * z), 0.01 * np.sin(z), 0.01 * np.cos(y)]
+    R, _ = o.eval_residual(*st)
+    return R, pm
+
+
+def test_empty_patch_skip_starts_after_the_first_boundary_face():
+    # the reference skips an "empty" face only if face > n_internal
+    # (Solver.cpp:308), so the first boundary face is still computed: on a
+    # one-cell-thick box with empty x sides (x0 holds boundary face 0) every
+    # residual equals the periodic-x one, except the owner of boundary face 0,
+    # which keeps that face's flux (the x pressure flux rho/3 A)
+    Re, pm = _thin_box_residual("empty")
+    Rp, _ = _thin_box_residual("periodic")
+    scale = np.abs(Rp).max()
+    d = np.abs(Re - Rp).max(1)
+    first = pm.owner[pm.n_internal]
+    others = np.arange(pm.n_cells) != first
+    assert d[others].max() < 1e-13 * scale
+    assert d[first] > scale
