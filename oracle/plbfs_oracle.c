@@ -64,12 +64,15 @@
  *                      the same face inside a longer box (types 1, 2, 6 do not);
  *   empty patches      "face > n_internal": on a one-cell-thick box with empty x
  *                      sides every residual equals the periodic-x one except
- *                      the owner of boundary face 0, which keeps that face's flux.
+ *                      the owner of boundary face 0, which keeps that face's flux;
+ *   "neighbour > n_cells" with y, z periodic and a quadratic density of x, every
+ *                      x = 0 face gives the same flux except boundary face 0,
+ *                      which takes the two-sided branch.
  * Parity unpinned (restated line by line from the cited source; no vector or
  * closed form distinguishes them from a plausible alternative): the value of
  * the reference LSQ determinant expression on non-axis-aligned stencils, the
- * "neighbour > n_cells" boundary test, the ghost gradients of type 6 (zero)
- * and the density slot of the type-8 ghost gradient (left zero).
+ * ghost gradients of type 6 (zero) and the density slot of the type-8 ghost
+ * gradient (left zero).
  */
 #include <math.h>
 #include <stdint.h>

@@ -19,7 +19,8 @@ restatement is pinned by what the mathematics fixes:
               keep the sign of n; symmetry-x ghosts mirror the owner and drag
               the stream toward the patch velocity; a type-7 density ghost
               continues the owner's gradient; empty patches are skipped from
-              the second boundary face on.
+              the second boundary face on, and boundary face 0 takes the
+              two-sided branch.
 """
 import numpy as np
 import pytest
@@ -561,3 +562,32 @@ def test_empty_patch_skip_starts_after_the_first_boundary_face():
     others = np.arange(pm.n_cells) != first
     assert d[others].max() < 1e-13 * scale
     assert d[first] > scale
+
+
+def test_first_boundary_face_takes_the_interior_branch():
+    # the reference treats a face as a boundary face only if its neighbour
+    # index is > n_cells (Solver.cpp:861); the ghost of boundary face 0 has
+    # index n_cells, so that face is reconstructed two-sidedly from the ghost
+    # state and gradient instead of taking the Dirichlet value.  With y and z
+    # periodic and a field of x alone, every x = 0 face sees the same cells;
+    # on a quadratic density the two-sided value differs from the patch value,
+    # so face 0's flux differs from the other x = 0 faces, which all agree
+    a, b, q = 1.0, 0.02, 0.004
+    pt = {"x0": "inl", "x1": "out", "y0": "ylo", "y1": "yhi", "z0": "zlo", "z1": "zhi"}
+    pm = meshgen.box_mesh(6, 3, 3, lengths=(6.0, 3.0, 3.0), permute_seed=5, patch_types=pt)
+    per = dict(rho=1.0, u=0.0, v=0.0, w=0.0, vel_type="periodic", rho_type="periodic")
+    bcs = {"inl": dict(rho=a, u=1.0, v=0.0, w=0.0, vel_type="dirichlet", rho_type="dirichlet"),
+           "out": dict(rho=1.0, u=0.0, v=0.0, w=0.0, vel_type="neumann", rho_type="neumann"),
+           "ylo": dict(per, per_translate=(0.0, 3.0, 0.0)), "yhi": dict(per, per_translate=(0.0, -3.0, 0.0)),
+           "zlo": dict(per, per_translate=(0.0, 0.0, 3.0)), "zhi": dict(per, per_translate=(0.0, 0.0, -3.0))}
+    case = pc.Case(mach_no=0.05, reynolds_no=40.0, ref_length=4.0, bcs=bcs)
+    o = oracle_for(pm, case)
+    x = o.cells()["cx"]
+    o.eval_residual(a + b * x + q * x * x, np.full(o.T, case.mach_no), np.zeros(o.T), np.zeros(o.T))
+    f = o.faces()
+    x0 = [k for k in range(pm.n_internal, pm.n_faces) if abs(f["fx"][k]) < 1e-9]
+    assert x0[0] == pm.n_internal and len(x0) == 9
+    F = np.array([o.face_flux(k) for k in x0])
+    scale = np.abs(F).max()
+    assert np.abs(F[1:] - F[1]).max() < 1e-13 * scale
+    assert np.abs(F[0] - F[1]).max() > 1e-6 * scale
