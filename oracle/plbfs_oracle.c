@@ -67,12 +67,15 @@
  *                      the owner of boundary face 0, which keeps that face's flux;
  *   "neighbour > n_cells" with y, z periodic and a quadratic density of x, every
  *                      x = 0 face gives the same flux except boundary face 0,
- *                      which takes the two-sided branch.
+ *                      which takes the two-sided branch;
+ *   LSQ determinant    on hexes sheared in x by y (LHS_xz = LHS_yz = 0) the
+ *                      reference inverse is the true one times
+ *                      (xx yy - xy^2)/(xx yy + xy^2): the sign of its xy term.
  * Parity unpinned (restated line by line from the cited source; no vector or
  * closed form distinguishes them from a plausible alternative): the value of
- * the reference LSQ determinant expression on non-axis-aligned stencils, the
- * ghost gradients of type 6 (zero) and the density slot of the type-8 ghost
- * gradient (left zero).
+ * the "- yz - zx" part of the reference LSQ determinant expression (it acts
+ * only where z couples to x or y), the ghost gradients of type 6 (zero) and
+ * the density slot of the type-8 ghost gradient (left zero).
  */
 #include <math.h>
 #include <stdint.h>

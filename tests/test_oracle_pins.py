@@ -20,7 +20,8 @@ restatement is pinned by what the mathematics fixes:
               the stream toward the patch velocity; a type-7 density ghost
               continues the owner's gradient; empty patches are skipped from
               the second boundary face on, and boundary face 0 takes the
-              two-sided branch.
+              two-sided branch; on x-by-y sheared hexes the reference LSQ
+              determinant flips the sign of its xy term.
 """
 import numpy as np
 import pytest
@@ -591,3 +592,24 @@ def test_first_boundary_face_takes_the_interior_branch():
     scale = np.abs(F).max()
     assert np.abs(F[1:] - F[1]).max() < 1e-13 * scale
     assert np.abs(F[0] - F[1]).max() > 1e-6 * scale
+
+
+def test_lsq_reference_determinant_on_sheared_cells():
+    # hexes sheared in x by y (x += 0.3 y) couple x and y in the LSQ matrix but
+    # leave z uncoupled (LHS_xz = LHS_yz = 0).  There the reference expression
+    # (gradients.cpp:282-284) reduces to zz (xx yy + xy^2) against the true
+    # zz (xx yy - xy^2), so its inverse is the true one scaled by
+    # s = (xx yy - xy^2) / (xx yy + xy^2) in every cell
+    pm = cavity_mesh(4, tets=False, jitter=0.0)
+    pts = pm.points.astype(np.float64)
+    pts[:, 0] += 0.3 * pts[:, 1]
+    pm.points = pts.astype(np.float32).astype(pm.points.dtype)
+    case = pc.cavity_case()
+    case.lsq_exact_det = 0
+    inv, L = _lhs_from_rhs(oracle_for(pm, case))
+    xx, yy, xy = L[:, 0, 0], L[:, 1, 1], L[:, 0, 1]
+    assert np.abs(L[:, 0, 2]).max() < 1e-12 * np.abs(L).max() and np.abs(L[:, 1, 2]).max() < 1e-12 * np.abs(L).max()
+    assert np.abs(xy).min() > 1e-3 * np.abs(L).max()
+    prod = inv @ L
+    s = (xx * yy - xy * xy) / (xx * yy + xy * xy)
+    assert np.abs(prod - s[:, None, None] * np.eye(3)).max() < 1e-9
