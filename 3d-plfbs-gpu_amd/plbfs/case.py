@@ -49,6 +49,10 @@ class Case:
     # uns_boundary_conditions/periodic_translation: one vector for every periodic
     # patch (src/preprocessor.cpp:299-301); a bc dict may override it per patch
     periodic_translation: tuple = (0.0, 0.0, 0.0)
+    # geometry x, y, z: the domain lengths that the parabolic inlet profiles
+    # (types 4 and 5) scale by, after multiplication by `scale`
+    # (src/preprocessor.cpp:48,467-469, src/Solution.cpp:804-813)
+    domain: tuple = (1.0, 1.0, 1.0)
 
     @property
     def factor(self):
@@ -71,7 +75,8 @@ class Case:
                     max_velocity=self.mach_no, ref_rho=self.average_rho, force=-self.pressure_gradient,
                     tolerance=self.tolerance, gradient_type=GRAD_CODES.get(self.gradient_calc, 2),
                     time_stepping=DT_CODES[self.time_stepping], precision=self.precision,
-                    check_interval=self.check_interval, lsq_exact_det=self.lsq_exact_det)
+                    check_interval=self.check_interval, lsq_exact_det=self.lsq_exact_det,
+                    domain_x=self.domain[0] * self.scale, domain_y=self.domain[1] * self.scale)
 
     def patches_for(self, patch_names):
         """Patch table in the mesh's patch order, matched by type token."""

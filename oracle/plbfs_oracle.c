@@ -70,12 +70,28 @@
  *                      which takes the two-sided branch;
  *   LSQ determinant    on hexes sheared in x by y (LHS_xz = LHS_yz = 0) the
  *                      reference inverse is the true one times
- *                      (xx yy - xy^2)/(xx yy + xy^2): the sign of its xy term.
- * Parity unpinned (restated line by line from the cited source; no vector or
- * closed form distinguishes them from a plausible alternative): the value of
- * the "- yz - zx" part of the reference LSQ determinant expression (it acts
- * only where z couples to x or y), the ghost gradients of type 6 (zero) and
- * the density slot of the type-8 ghost gradient (left zero).
+ *                      (xx yy - xy^2)/(xx yy + xy^2): the sign of its xy term;
+ *                      sheared in x by y and z (LHS_yz = 0), the reference
+ *                      expression is zz (xx yy + xy^2) - xy xz - yy xz^2: the
+ *                      "- yz - zx" part (tests/test_oracle_flux_pins.py);
+ *   face_flux (non-equilibrium part, tests/test_oracle_flux_pins.py)
+ *                      a linear shear on uniform hexes gives the closed-form
+ *                      moments (tau' = 3 nu/(dt gamma), offsets -e.grad dt,
+ *                      1/gamma in feq) at gamma 1 and 0.3; on jittered hexes
+ *                      and tets every interior face equals the flux written
+ *                      from its definition (lattice point x_f - e dt from the
+ *                      side of the face plane it lies on, rest point the mean
+ *                      of the two face values), and on unjittered hexes points
+ *                      on the face plane take the owner side;
+ *   ghost gradients    inlet (6) faces: zero; symmetry-x (8) faces: zero
+ *                      density slot, Dirichlet velocity formula (the boundary
+ *                      flux equals the definition with these gradients);
+ *   parabolic ghosts   types 4 and 5 at the centroid of cell b (b the boundary
+ *                      index) on an unpermuted box, u of type 5 with (X - y),
+ *                      w and type 9 at the initial state.
+ * Every pin in tests/test_oracle_flux_pins.py also rejects a copy of this file
+ * built with a planted mistake (OM_PLANT below).  No function is left
+ * unpinned.
  */
 #include <math.h>
 #include <stdint.h>
@@ -87,6 +103,21 @@
 #define OM_ERR_NAN -2
 #define OM_ERR_RHO -3
 #define OM_ERR_BCTYPE -4
+
+/* Planted mistakes, for the tests that show each pin fails when the
+ * restatement is wrong in a plausible way.  The library the tests use as the
+ * oracle is built with OM_PLANT 0 (none); oracle/Makefile's `plants` target
+ * builds one library per mistake:
+ *   1 tau' = 3 nu / dt without the 1/gamma      2 tau' doubled
+ *   3 lattice offsets +e.grad dt (sign flipped)  4 lattice side flipped (neighbour iff e.n > 0)
+ *   5 rest point from the owner side only        6 LSQ determinant "- yz * zx" for "- yz - zx"
+ *   7 type-6 ghost gradients copy the owner's    8 type-8 density ghost gradient copies the owner's
+ *   9 parabolic ghosts at the owner's centroid instead of cell i's
+ *  10 type-5 ghost u with (X - x) for (X - y)
+ *  11 lattice points on the face plane (e.n = 0) from the neighbour side */
+#ifndef OM_PLANT
+#define OM_PLANT 0
+#endif
 
 /* ------------------------------------------------------------------------- */
 /* Inputs                                                                    */
@@ -121,6 +152,7 @@ typedef struct {
     int gradient_type;             /* 0 LS, 1 hybrid, 2 Green-Gauss */
     int time_stepping;             /* 0 local, 1 constant, 2 min, 3 talts */
     int lsq_exact_det;             /* 0: reference determinant expression; 1: true determinant */
+    double domain_x, domain_y;     /* domain.X, domain.Y (geometry x, y times scale, preprocessor.cpp:48,467-468) */
 } om_params;
 
 /* ------------------------------------------------------------------------- */
@@ -332,6 +364,7 @@ static void build_lsq(om_ctx *c) {
             }
         }
         double det = xx * (yy * zz - yz * zy) + xy * (yx * zz - yz - zx) + xz * (yx * zy - yy * zx);
+        if (OM_PLANT == 6) det = xx * (yy * zz - yz * zy) + xy * (yx * zz - yz * zx) + xz * (yx * zy - yy * zx);
         if (c->p.lsq_exact_det) det = xx * (yy * zz - yz * zy) - xy * (yx * zz - yz * zx) + xz * (yx * zy - yy * zx);
         double a = xx, b = xy, cc = xz, d = yx, e = yy, ff = yz, g = zx, h = zy, k = zz;
         c->Lxx[i] = 1 / det * (e * k - ff * h);
@@ -396,11 +429,13 @@ static void build_partners(om_ctx *c) {
 /* Creation                                                                  */
 /* ------------------------------------------------------------------------- */
 
+/* Type codes 1..9 of preprocessor.cpp:243-262.  Codes the ghost update does
+ * not write (density 4, 5, 9; velocity 7, 9, and w of 4 and 5) leave the
+ * ghost at its value in soln, which no step writes: the initial state. */
 static int check_patch_types(const om_patch *p, int n) {
     for (int i = 0; i < n; i++) {
         int r = p[i].rho_type, v = p[i].vel_type;
-        if (!(r == 1 || r == 2 || r == 3 || r == 6 || r == 7 || r == 8)) return OM_ERR_BCTYPE;
-        if (!(v == 1 || v == 2 || v == 3 || v == 6 || v == 8)) return OM_ERR_BCTYPE;
+        if (r < 1 || r > 9 || v < 1 || v > 9) return OM_ERR_BCTYPE;
     }
     return OM_OK;
 }
@@ -574,6 +609,19 @@ static void update_bcs(om_ctx *c, double *rho, double *u, double *v, double *w) 
             u[j] = u[pn];
             v[j] = v[pn];
             w[j] = w[pn];
+        } else if (P->vel_type == 4) {
+            /* parabolic-N (Solution.cpp:804-808): the profile is evaluated at
+             * the centroid of cell b (the boundary index), w is not written */
+            int ci = (OM_PLANT == 9) ? nb : b;
+            double Y = c->p.domain_y;
+            u[j] = 4 * P->u / pow(Y, 2) * c->cy[ci] * (Y - c->cy[ci]);
+            v[j] = 4 * P->v / pow(Y, 2) * c->cy[ci] * (Y - c->cy[ci]);
+        } else if (P->vel_type == 5) {
+            /* parabolic-W (Solution.cpp:809-813): u takes x (X - y) of cell b */
+            int ci = (OM_PLANT == 9) ? nb : b;
+            double X = c->p.domain_x;
+            u[j] = 4 * P->u / pow(X, 2) * c->cx[ci] * (X - ((OM_PLANT == 10) ? c->cx[ci] : c->cy[ci]));
+            v[j] = 4 * P->v / pow(X, 2) * c->cx[ci] * (X - c->cx[ci]);
         } else if (P->vel_type == 6) {
             double taper = 1.0;
             u[j] = P->u * taper;
@@ -645,10 +693,14 @@ static void compute_gradients(om_ctx *c, const double *rho, const double *u, con
             GI(j, 2) = GI(o, 2) * (1 - c->nk[f]);
         } else if (P->rho_type == 6) {
             GI(j, 0) = 0.0; GI(j, 1) = 0.0; GI(j, 2) = 0.0;
+            if (OM_PLANT == 7) { GI(j, 0) = GI(o, 0); GI(j, 1) = GI(o, 1); GI(j, 2) = GI(o, 2); }
         } else if (P->rho_type == 7) {
             GI(j, 0) = GI(o, 0); GI(j, 1) = GI(o, 1); GI(j, 2) = GI(o, 2);
         } else if (P->rho_type == 8) {
+            /* the velocity formula; the density slots are never written
+             * (gradients.cpp:429-431): zero, as in a freshly allocated array */
             ghost_grads_dirichlet(c, rho, u, v, w, b, o, 0);
+            if (OM_PLANT == 8) { GI(j, 0) = GI(o, 0); GI(j, 1) = GI(o, 1); GI(j, 2) = GI(o, 2); }
         } else {
             /* periodic: the partner's gradient (gradients.cpp:434-440) */
             const int pn = c->partner[b];
@@ -663,7 +715,7 @@ static void compute_gradients(om_ctx *c, const double *rho, const double *u, con
                 GI(j, 3 * q + 2) = GI(o, 3 * q + 2) * (1 - c->nk[f]);
             }
         } else if (P->vel_type == 6) {
-            for (int q = 3; q < 12; q++) GI(j, q) = 0.0;
+            for (int q = 3; q < 12; q++) GI(j, q) = (OM_PLANT == 7) ? GI(o, q) : 0.0;
         } else if (P->vel_type == 8) {
             ghost_grads_dirichlet(c, rho, u, v, w, b, o, 0);
         } else {
@@ -712,9 +764,11 @@ static void face_flux(om_ctx *c, int face, const double *rho, const double *u, c
         rho_i = rho[i] - (t1x * gi[0] + t1y * gi[1] + t1z * gi[2]);
         rho_nb = rho[nb] - (t2x * gn[0] + t2y * gn[1] + t2z * gn[2]);
         lr[0] = (rho_i + rho_nb) * 0.5;
+        if (OM_PLANT == 5) lr[0] = rho_i;
         u_i = u[i] - (t1x * gi[3] + t1y * gi[4] + t1z * gi[5]);
         u_nb = u[nb] - (t2x * gn[3] + t2y * gn[4] + t2z * gn[5]);
         lu[0] = (u_i + u_nb) * 0.5;
+        if (OM_PLANT == 5) lu[0] = u_i;
         v_i = v[i] - (t1x * gi[6] + t1y * gi[7] + t1z * gi[8]);
         v_nb = v[nb] - (t2x * gn[6] + t2y * gn[7] + t2z * gn[8]);
         lv[0] = (v_i + v_nb) * 0.5;
@@ -726,7 +780,7 @@ static void face_flux(om_ctx *c, int face, const double *rho, const double *u, c
      * as the reference's sum of signed normal components > 0 (Solver.cpp:958-1273) */
     for (int k = 1; k < 15; k++) {
         double cond = (-EX[k]) * n[0] + (-EY[k]) * n[1] + (-EZ[k]) * n[2];
-        int use_nb = cond > 0;
+        int use_nb = (OM_PLANT == 4) ? cond < 0 : (OM_PLANT == 11) ? cond >= 0 : cond > 0;
         const double *gg = use_nb ? gn : gi;
         double br = use_nb ? rho_nb : rho_i, bu = use_nb ? u_nb : u_i;
         double bv = use_nb ? v_nb : v_i, bw = use_nb ? w_nb : w_i;
@@ -734,6 +788,7 @@ static void face_flux(om_ctx *c, int face, const double *rho, const double *u, c
         double vals[4] = {br, bu, bv, bw};
         for (int q = 0; q < 4; q++) {
             double r = vals[q];
+            if (OM_PLANT == 3) r = r + 2 * (EX[k] * gg[3 * q + 0] + EY[k] * gg[3 * q + 1] + EZ[k] * gg[3 * q + 2]) * dt;
             if (EX[k] > 0) r = r - gg[3 * q + 0] * dt; else if (EX[k] < 0) r = r + gg[3 * q + 0] * dt;
             if (EY[k] > 0) r = r - gg[3 * q + 1] * dt; else if (EY[k] < 0) r = r + gg[3 * q + 1] * dt;
             if (EZ[k] > 0) r = r - gg[3 * q + 2] * dt; else if (EZ[k] < 0) r = r + gg[3 * q + 2] * dt;
@@ -781,6 +836,8 @@ static void face_flux(om_ctx *c, int face, const double *rho, const double *u, c
     double uu = uIx, vv = uIy, ww = uIz;
     double uv = uu * vv * 9.0 / gam, uw = uu * ww * 9.0 / gam, vw = vv * ww * 9.0 / gam;
     double tau = (c->p.visc * 3 / dt / gam);
+    if (OM_PLANT == 1) tau = c->p.visc * 3 / dt;
+    if (OM_PLANT == 2) tau = 2 * tau;
     double fi[15];
     for (int k = 1; k < 15; k++) {
         double t;
@@ -906,6 +963,14 @@ void om_eval_residual(om_ctx *c, const double *rho, const double *u, const doubl
  * (call om_eval_residual first). */
 void om_face_flux(om_ctx *c, int face, double *out4) {
     face_flux(c, face, c->t_rho, c->t_u, c->t_v, c->t_w, out4, NULL);
+}
+
+/* Exposed for unit tests: the temp state over T cells after om_eval_residual
+ * (interior cells as given, ghosts from the boundary update). */
+void om_get_temp(const om_ctx *c, double *rho, double *u, double *v, double *w) {
+    size_t T = (size_t)c->T;
+    memcpy(rho, c->t_rho, sizeof(double) * T); memcpy(u, c->t_u, sizeof(double) * T);
+    memcpy(v, c->t_v, sizeof(double) * T); memcpy(w, c->t_w, sizeof(double) * T);
 }
 
 void om_get_dt(om_ctx *c, double *dt) {
