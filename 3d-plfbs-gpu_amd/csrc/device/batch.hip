@@ -182,8 +182,12 @@ __global__ __launch_bounds__(kFinThreads) void kb_finalize(int nblk, int B, doub
         for (int c = 0; c < 4; c++) cc.res[(size_t)b * 4 + c] = r[c];
         cc.steps[b] += 1;
         const int e = (int)red[0][4];
-        if (e == 1) cc.status[b] = PLBFS_ERR_NAN;
-        else if (e == 2) cc.status[b] = PLBFS_ERR_RHO;
+        // a status set earlier in the step (the immersed nodes) is the cause
+        // and is kept; NaN or rho blow-up is reported only on its own
+        if (cc.status[b] == PLBFS_OK) {
+            if (e == 1) cc.status[b] = PLBFS_ERR_NAN;
+            else if (e == 2) cc.status[b] = PLBFS_ERR_RHO;
+        }
         if (cc.status[b] != PLBFS_OK) {
             cc.done[b] = 1;
         } else if (fmax(fmax(r[3], r[1]), r[2]) < tol) {
@@ -331,33 +335,43 @@ size_t lds_bytes_for(const plbfs::Layout &L, int C, bool f32, bool stage) {
 }
 
 // (Re)builds the shared layout for a flux block of block_cells cells and C
-// cases, staging the tile's records in LDS when stage is set.
+// cases, staging the tile's records in LDS when stage is set.  The new layout
+// is built beside the current one and replaces it only once it is complete:
+// a tiling that is refused (LDS, allocation) leaves the batch as it was.
 int retile(plbfs_batch *b, int block_cells, int C, int stage) {
     if (block_cells < 64 || block_cells > 1024 || block_cells % 64)
         return plbfs_fail(PLBFS_ERR_INPUT, "block_cells %d: need a multiple of 64 in [64, 1024]", block_cells);
     if (C < 1 || C > kBlock || kBlock % C)
         return plbfs_fail(PLBFS_ERR_INPUT, "cases_per_block %d: need a divisor of %d", C, kBlock);
-    drop_graph(b);
-    if (b->base) {
-        plbfs_destroy(b->base);
-        b->base = nullptr;
-    }
     plbfs_mesh_view v = view_of(b);
+    plbfs_ctx *nb = nullptr;
     int rc = create_ctx(&v, &b->params, b->patches.data(), (int32_t)b->patches.size(),
-                        b->bpatch.empty() ? nullptr : b->bpatch.data(), block_cells, C, false, false, &b->base);
+                        b->bpatch.empty() ? nullptr : b->bpatch.data(), block_cells, C, false, false, &nb);
     if (rc) return rc;
+    const size_t lds = lds_bytes_for(nb->L, C, b->f32, stage != 0);
+    if (lds > (size_t)kMaxLds) {
+        plbfs_destroy(nb);
+        return plbfs_fail(PLBFS_ERR_INPUT, "tile of %d cells x %d cases needs %zu B of LDS (limit %d)", block_cells,
+                          C, lds, kMaxLds);
+    }
+    nb->lds_bytes = lds;
+    const size_t need = (size_t)nb->L.n_blocks * b->B * 5;
+    if (need > b->partials_cap) {
+        // the old array stays in b->allocs (freed with the batch): queued work may still read it
+        double *np = nullptr;
+        if ((rc = balloc(b, (void **)&np, need * sizeof(double)))) {
+            plbfs_destroy(nb);
+            return rc;
+        }
+        b->partials = np;
+        b->partials_cap = need;
+    }
+    drop_graph(b);
+    if (b->base) plbfs_destroy(b->base);
+    b->base = nb;
     b->block_cells = block_cells;
     b->C = C;
     b->stage = stage;
-    b->base->lds_bytes = lds_bytes_for(b->base->L, C, b->f32, stage != 0);
-    if (b->base->lds_bytes > (size_t)kMaxLds)
-        return plbfs_fail(PLBFS_ERR_INPUT, "tile of %d cells x %d cases needs %zu B of LDS (limit %d)", block_cells,
-                          C, b->base->lds_bytes, kMaxLds);
-    const size_t need = (size_t)b->base->L.n_blocks * b->B * 5;
-    if (need > b->partials_cap) {
-        if ((rc = balloc(b, (void **)&b->partials, need * sizeof(double)))) return rc;
-        b->partials_cap = need;
-    }
     return PLBFS_OK;
 }
 
@@ -529,13 +543,17 @@ int build_batch_graph(plbfs_batch *b) {
     return PLBFS_OK;
 }
 
-int reset_case_ctl(plbfs_batch *b) {
-    const size_t B = b->B;
-    HIP_TRY(hipMemset(b->done, 0, B * sizeof(int32_t)));
-    HIP_TRY(hipMemset(b->status, 0, B * sizeof(int32_t)));
-    HIP_TRY(hipMemset(b->steps, 0, B * sizeof(int32_t)));
-    HIP_TRY(hipMemset(b->conv, 0, B * sizeof(int32_t)));
-    HIP_TRY(hipMemset(b->res, 0, B * 4 * sizeof(double)));
+// Clears step count, convergence, status and residual of case case_id (every
+// case when case_id < 0).  The other cases keep theirs: get_state reads a
+// case's step count to know whether its t0 holds primitives or conservatives.
+int reset_case_ctl(plbfs_batch *b, int case_id = -1) {
+    const size_t k0 = case_id < 0 ? 0 : (size_t)case_id;
+    const size_t n = case_id < 0 ? (size_t)b->B : 1;
+    HIP_TRY(hipMemset(b->done + k0, 0, n * sizeof(int32_t)));
+    HIP_TRY(hipMemset(b->status + k0, 0, n * sizeof(int32_t)));
+    HIP_TRY(hipMemset(b->steps + k0, 0, n * sizeof(int32_t)));
+    HIP_TRY(hipMemset(b->conv + k0, 0, n * sizeof(int32_t)));
+    HIP_TRY(hipMemset(b->res + 4 * k0, 0, n * 4 * sizeof(double)));
     return PLBFS_OK;
 }
 
@@ -691,7 +709,7 @@ extern "C" int plbfs_batch_set_state(plbfs_batch *b, int32_t case_id, const doub
                            (const double4 *)b->scratch, (double4 *)b->t0, (double4 *)b->stA);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(s));
-    return reset_case_ctl(b);
+    return reset_case_ctl(b, case_id);
 }
 
 extern "C" int plbfs_batch_get_state(plbfs_batch *b, int32_t case_id, double *rho, double *u, double *v,

@@ -93,7 +93,10 @@ __global__ __launch_bounds__(kIbmThreads) void k_ibm_nodes(IbmArgs a, Ctrl *__re
     su = wave_allsum(su);
     sv = wave_allsum(sv);
     sw = wave_allsum(sw);
-    const double U[3] = {su / S, sv / S, sw / S};
+    // a node with no cell in its support (S = 0) is an error below; it stays
+    // where it is, so that only finite positions and forces are written
+    const bool supported = S > 0.0;
+    const double U[3] = {supported ? su / S : 0.0, supported ? sv / S : 0.0, supported ? sw / S : 0.0};
     const double kdA = a.node_kdA ? a.node_kdA[k] : a.k_dA;
     double Xn[3], F[3], drift = 0;
 #pragma unroll
@@ -120,7 +123,7 @@ __global__ __launch_bounds__(kIbmThreads) void k_ibm_nodes(IbmArgs a, Ctrl *__re
             a.force[(size_t)q * n + k] = F[q];
         }
         a.norm[k] = S2;
-        if (!(S > 0.0) || !(S2 > 0.0) || drift > a.margin) {
+        if (!supported || !(S2 > 0.0) || drift > a.margin) {
             if (a.case_status) {
                 // batch: only this case stops
                 a.case_status[b] = PLBFS_ERR_IBM;

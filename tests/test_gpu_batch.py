@@ -169,3 +169,76 @@ def test_autotune_keeps_results(gpu_available):
     for b in range(4):
         for q in range(4):
             assert np.array_equal(bt.state(b)[q], ref.state(b)[q]), (b, q)
+
+
+def test_set_state_of_one_case_keeps_the_others(gpu_available):
+    # re-seeding case 1 must leave case 0's state, step count and RK history alone
+    pm = channel_mesh(10, 5, 5)
+    cases = cases_for(RES)
+    o = oracle_for(pm, cases[0])
+    st = perturbed_state(cases[0], o, amp=0.01, seed=3)
+    st1 = perturbed_state(cases[0], o, amp=0.02, seed=4)
+    hm, bt = make_batch(pm, cases)
+    bt.set_state(-1, *st)
+    assert bt.run(10)[0] == 0
+    before = bt.state(0)
+    bt.set_state(1, *st1)
+    after = bt.state(0)
+    for q in range(4):
+        assert np.array_equal(before[q], after[q]), q
+    rc, res = bt.run(10)
+    assert rc == 0 and [r["steps"] for r in res] == [20, 10, 20], res
+    for b, (start, n) in enumerate([(st, 20), (st1, 10), (st, 20)]):
+        r1, s1 = single_run(pm, cases[b], start, n)
+        for q in range(4):
+            assert np.array_equal(bt.state(b)[q], s1[q]), (b, q)
+
+
+def test_refused_tiling_leaves_the_batch_runnable(gpu_available):
+    # 1024 cells x 8 FP64 cases cannot fit the LDS: the call fails and the
+    # batch keeps its tiling and gives the same fields as an untouched batch
+    pm = channel_mesh(12, 6, 6)
+    cases = cases_for(RES)
+    o = oracle_for(pm, cases[0])
+    st = perturbed_state(cases[0], o, amp=0.01, seed=9)
+    hm, bt = make_batch(pm, cases)
+    bt.set_tiling(128, 1, 0)
+    info = bt.layout_info()
+    with pytest.raises(RuntimeError):
+        bt.set_tiling(1024, 8, 0)
+    assert bt.layout_info() == info
+    bt.set_state(-1, *st)
+    assert bt.run(15)[0] == 0
+    hm2, ref = make_batch(pm, cases)
+    ref.set_tiling(128, 1, 0)
+    ref.set_state(-1, *st)
+    assert ref.run(15)[0] == 0
+    for b in range(len(cases)):
+        for q in range(4):
+            assert np.array_equal(bt.state(b)[q], ref.state(b)[q]), (b, q)
+
+
+def test_node_outside_its_support_stops_only_its_case(gpu_available):
+    # a node 2.2 h outside the inlet: candidate cells exist (within support +
+    # margin) but every Peskin weight is zero, so its case stops with
+    # PLBFS_ERR_IBM (not NaN) and the other cases run on
+    pm = channel_mesh(16, 8, 8)
+    cases = cases_for([40.0, 80.0, 20.0])
+    o = oracle_for(pm, cases[0])
+    st = initial_state(cases[0], o)
+    hm, bt = make_batch(pm, cases)
+    bt.set_state(-1, *st)
+    X, dA = objects.fibonacci_sphere(60, (6.0, 4.0, 4.0), 1.5)
+    ibm = dict(x_ref=X, x=X, stiffness=2.0, node_area=dA, dt=cases[0].time_marching_step, h=1.0, kernel=1)
+    Xs = np.vstack([X, [[-2.2, 4.0, 4.0]]])
+    bt.set_ibm(0, **ibm)
+    bt.set_ibm(1, **dict(ibm, x_ref=Xs, x=Xs))
+    rc, res = bt.run(10)
+    assert rc == -7 and res[1]["status"] == -7 and res[1]["steps"] == 1, res
+    assert res[0]["status"] == 0 and res[2]["status"] == 0 and res[0]["steps"] == 10 == res[2]["steps"], res
+    assert all(np.all(np.isfinite(a)) for a in bt.state(1))
+    r0, s0 = single_run(pm, cases[0], st, 10, ibm=ibm)
+    r2, s2 = single_run(pm, cases[2], st, 10)
+    for q in range(4):
+        assert np.array_equal(bt.state(0)[q], s0[q]), q
+        assert np.array_equal(bt.state(2)[q], s2[q]), q
