@@ -86,6 +86,9 @@ struct plbfs_ctx {
     hipGraphExec_t graph_exec = nullptr;
     bool graph_ok = false;
     bool use_graph = true;
+    // this device's copy of the file-scope scratch array of solver.hip
+    // (plbfs_run_noalloc), resolved once per device at creation
+    char *scratch = nullptr;
 };
 
 inline int dalloc(std::vector<void *> &list, void **p, size_t bytes) {
@@ -115,16 +118,17 @@ inline int upload(plbfs_ctx *c, T **dst, const std::vector<T> &src) {
 struct TimedLaunch {
     plbfs_ctx *c;
     int which;
+    bool on;
     hipEvent_t a = nullptr, b = nullptr;
-    TimedLaunch(plbfs_ctx *cc, int w) : c(cc), which(w) {
-        if (c->timing) {
+    TimedLaunch(plbfs_ctx *cc, int w, bool timed) : c(cc), which(w), on(timed) {
+        if (on) {
             (void)hipEventCreate(&a);
             (void)hipEventCreate(&b);
             (void)hipEventRecord(a, c->stream);
         }
     }
     ~TimedLaunch() {
-        if (c->timing) {
+        if (on) {
             (void)hipEventRecord(b, c->stream);
             c->pending.push_back({which, a, b});
         }
@@ -139,8 +143,9 @@ int create_ctx(const plbfs_mesh_view *mesh, const plbfs_params *params, const pl
                int32_t n_patches, const int32_t *bface_patch, int block_cells, int lds_cases, bool flux_stage,
                bool split_border, plbfs_ctx **out);
 
-// Launches the per-step immersed-boundary kernels (ibm.hip); no-op when off.
-int ibm_launch(plbfs_ctx *c);
+// Launches the per-step immersed-boundary kernels (ibm.hip) against the control
+// block ctrl; no-op when off.  timed: bracket them with timing events.
+int ibm_launch(plbfs_ctx *c, plbfs::Ctrl *ctrl, bool timed);
 
 // Node-major candidate cells of every node (centroid within the kernel support
 // plus a drift margin of the anchor) and the same pairs regrouped by cell.

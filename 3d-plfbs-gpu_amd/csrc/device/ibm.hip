@@ -66,18 +66,18 @@ IbmArgs make_args(plbfs_ctx *c) {
 
 }  // namespace
 
-int ibm_launch(plbfs_ctx *c) {
+int ibm_launch(plbfs_ctx *c, Ctrl *ctrl, bool timed) {
     if (!c->ibm.on) return PLBFS_OK;
     const IbmArgs a = make_args(c);
-    TimedLaunch t(c, PLBFS_KSTAT_IBM);
+    TimedLaunch t(c, PLBFS_KSTAT_IBM, timed);
     const dim3 grid((a.n + kNodesPerBlock - 1) / kNodesPerBlock);
     if (c->f32)
-        hipLaunchKernelGGL(k_ibm_nodes<float4>, grid, dim3(kIbmThreads), 0, c->stream, a, c->ctrl, (const float4 *)c->stA);
+        hipLaunchKernelGGL(k_ibm_nodes<float4>, grid, dim3(kIbmThreads), 0, c->stream, a, ctrl, (const float4 *)c->stA);
     else
-        hipLaunchKernelGGL(k_ibm_nodes<double4>, grid, dim3(kIbmThreads), 0, c->stream, a, c->ctrl,
+        hipLaunchKernelGGL(k_ibm_nodes<double4>, grid, dim3(kIbmThreads), 0, c->stream, a, ctrl,
                            (const double4 *)c->stA);
     hipLaunchKernelGGL(k_ibm_spread, dim3((a.n_affected + kNodesPerBlock - 1) / kNodesPerBlock), dim3(kIbmThreads), 0, c->stream, a,
-                       c->ctrl, c->fbody);
+                       ctrl, c->fbody);
     HIP_TRY(hipGetLastError());
     return PLBFS_OK;
 }

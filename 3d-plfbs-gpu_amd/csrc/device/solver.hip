@@ -19,9 +19,11 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -341,6 +343,36 @@ __global__ __launch_bounds__(kBlock) void k_residual_only(DevMesh<S> m, DevParam
 // Context
 // ---------------------------------------------------------------------------
 
+// ---- the time loop without allocation --------------------------------------
+// plbfs_run_noalloc works in this array instead of the context's control block,
+// partials, dt minima and history; it allocates nothing, builds no graph and
+// creates no events.  Each device holds its own copy until the process ends;
+// kScratchBytes bounds what one call may use.  Calls that overlap on one
+// device share the array and race.
+constexpr size_t kScratchBytes = (size_t)8 << 20;
+static __device__ __align__(256) double scratch[kScratchBytes / sizeof(double)];
+
+constexpr int kMaxScratchDevices = 64;
+static std::mutex g_scratch_mu;
+static char *g_scratch_ptr[kMaxScratchDevices];
+
+// This device's address of the scratch array; called by create_ctx with the
+// context's device current, once per device.
+static int resolve_scratch(int device, char **out) {
+    if (device < 0 || device >= kMaxScratchDevices)
+        return plbfs_fail(PLBFS_ERR_HIP, "device %d: scratch table holds %d devices", device, kMaxScratchDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return plbfs_fail(PLBFS_ERR_HIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+        g_scratch_ptr[device] = static_cast<char *>(p);
+    }
+    *out = g_scratch_ptr[device];
+    return PLBFS_OK;
+}
+
 static int release(plbfs_ctx *c) {
     if (c->graph_exec) {
         hipGraphExecDestroy(c->graph_exec);
@@ -431,6 +463,7 @@ int create_ctx(const plbfs_mesh_view *mesh, const plbfs_params *params, const pl
         return r;
     };
     if (hipGetDevice(&c->device) != hipSuccess) return fail(plbfs_fail(PLBFS_ERR_HIP, "hipGetDevice"));
+    if ((rc = resolve_scratch(c->device, &c->scratch))) return fail(rc);
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
         return fail(plbfs_fail(PLBFS_ERR_HIP, "hipStreamCreate"));
     const int N = L.N;
@@ -672,57 +705,79 @@ static void resolve_timing(plbfs_ctx *c) {
 static const double kAlpha[4] = {1.0, 0.5, 0.5, 1.0};
 static const double kBeta[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
 
-static int launch_dt_modes(plbfs_ctx *c) {
+// Device memory a time loop works in besides the context's fields: the control
+// block, per-block residual partials, per-block dt minima and the residual
+// history.  plbfs_run uses the context's own arrays, plbfs_run_noalloc carves
+// them out of the file-scope scratch array.
+struct RunScratch {
+    Ctrl *ctrl;
+    double *partials;  // [n_blocks * 5]
+    double *bmin;      // [cell_grid]
+    double *hist;      // [hist_cap * 4]
+    int hist_cap;
+};
+
+static RunScratch ctx_scratch(plbfs_ctx *c) { return RunScratch{c->ctrl, c->partials, c->bmin, c->hist, c->hist_cap}; }
+
+static void launch_dt_modes(plbfs_ctx *c, const RunScratch &w) {
     const int N = c->L.N, nb = c->cell_grid;
     if (c->params.time_stepping == PLBFS_DT_MIN || c->params.time_stepping == PLBFS_DT_TALTS) {
-        hipLaunchKernelGGL(k_dt_blockmin, dim3(nb), dim3(kBlock), 0, c->stream, N, c->ctrl, c->dtl, c->bmin);
+        hipLaunchKernelGGL(k_dt_blockmin, dim3(nb), dim3(kBlock), 0, c->stream, N, w.ctrl, c->dtl, w.bmin);
         hipLaunchKernelGGL(k_dt_apply, dim3(nb), dim3(kBlock), 0, c->stream, N, nb, c->params.time_stepping,
-                           c->ctrl, c->bmin, c->dtl);
+                           w.ctrl, w.bmin, c->dtl);
     }
-    return PLBFS_OK;
 }
 
 template <typename S>
-static void launch_stages(plbfs_ctx *c, const DevMesh<S> &dm) {
+static void launch_stages(plbfs_ctx *c, const DevMesh<S> &dm, const RunScratch &w, bool timed) {
     const int nb = c->L.n_blocks;
     void *in = c->stA, *outb = c->stB;
     for (int k = 0; k < 4; k++) {
         {
-            TimedLaunch t(c, PLBFS_KSTAT_GRAD);
-            hipLaunchKernelGGL(k_grad<S>, dim3(c->cell_grid), dim3(kBlock), 0, c->stream, dm, c->ctrl, (const V4<S> *)in,
+            TimedLaunch t(c, PLBFS_KSTAT_GRAD, timed);
+            hipLaunchKernelGGL(k_grad<S>, dim3(c->cell_grid), dim3(kBlock), 0, c->stream, dm, w.ctrl, (const V4<S> *)in,
                                (V4<S> *)c->grad);
         }
         StageArgs sa{kAlpha[k], kBeta[k], k};
         {
-            TimedLaunch t(c, PLBFS_KSTAT_FLUX);
+            TimedLaunch t(c, PLBFS_KSTAT_FLUX, timed);
             if (dm.n_border > 0)
                 hipLaunchKernelGGL(k_flux_border<S>, dim3((dm.n_border + kBlock - 1) / kBlock), dim3(kBlock), 0,
-                                   c->stream, dm, c->prm, c->ctrl, (const V4<S> *)in, (const V4<S> *)c->grad);
+                                   c->stream, dm, c->prm, w.ctrl, (const V4<S> *)in, (const V4<S> *)c->grad);
             hipLaunchKernelGGL(k_flux_update<S>, dim3(nb), dim3(kBlock), c->lds_bytes, c->stream, dm, c->prm, sa,
-                               c->ctrl, (const V4<S> *)in, (const V4<S> *)c->grad, (V4<S> *)outb,
-                               (double4 *)c->t0, (Acc<S> *)c->t1, c->dtl, c->fbody, c->partials);
+                               w.ctrl, (const V4<S> *)in, (const V4<S> *)c->grad, (V4<S> *)outb,
+                               (double4 *)c->t0, (Acc<S> *)c->t1, c->dtl, c->fbody, w.partials);
         }
         std::swap(in, outb);
     }
 }
 
 template <typename S>
-static void launch_init_dt(plbfs_ctx *c, const DevMesh<S> &dm) {
-    hipLaunchKernelGGL(k_init_dt<S>, dim3(c->cell_grid), dim3(kBlock), 0, c->stream, c->L.N, c->prm, c->ctrl,
+static void launch_init_dt(plbfs_ctx *c, const DevMesh<S> &dm, const Ctrl *ctrl) {
+    hipLaunchKernelGGL(k_init_dt<S>, dim3(c->cell_grid), dim3(kBlock), 0, c->stream, c->L.N, c->prm, ctrl,
                        dm.cfl_area, dm.vol, (const V4<S> *)c->stA, c->dtl);
 }
 
-static int launch_step(plbfs_ctx *c) {
+// The loop's first launches: the local time steps of the current state.
+static int launch_run_start(plbfs_ctx *c, const RunScratch &w) {
+    if (c->f32) launch_init_dt<float>(c, c->dm32, w.ctrl);
+    else launch_init_dt<double>(c, c->dm, w.ctrl);
+    launch_dt_modes(c, w);
+    HIP_TRY(hipGetLastError());
+    return PLBFS_OK;
+}
+
+static int launch_step(plbfs_ctx *c, const RunScratch &w, bool timed) {
     const int nb = c->L.n_blocks;
-    int rc = ibm_launch(c);  // once per step, before the stages (Solver_gpu.cu:687-718)
+    int rc = ibm_launch(c, w.ctrl, timed);  // once per step, before the stages (Solver_gpu.cu:687-718)
     if (rc) return rc;
-    if (c->f32) launch_stages<float>(c, c->dm32);
-    else launch_stages<double>(c, c->dm);
-    launch_dt_modes(c);
+    if (c->f32) launch_stages<float>(c, c->dm32, w, timed);
+    else launch_stages<double>(c, c->dm, w, timed);
+    launch_dt_modes(c, w);
     {
-        TimedLaunch t(c, PLBFS_KSTAT_REDUCE);
+        TimedLaunch t(c, PLBFS_KSTAT_REDUCE, timed);
         hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kFinalizeThreads), 0, c->stream, nb, (double)c->L.N,
-                           c->prm.ref_rho, c->prm.ref_mom, c->prm.tol, c->ctrl, c->partials, c->hist, c->hist_cap);
+                           c->prm.ref_rho, c->prm.ref_mom, c->prm.tol, w.ctrl, w.partials, w.hist, w.hist_cap);
     }
     HIP_TRY(hipGetLastError());
     return PLBFS_OK;
@@ -746,7 +801,7 @@ static int build_graph(plbfs_ctx *c) {
     if (c->graph_ok || !c->use_graph || c->timing) return PLBFS_OK;
     hipGraph_t g;
     HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    int rc = launch_step(c);
+    int rc = launch_step(c, ctx_scratch(c), false);
     hipError_t e = hipStreamEndCapture(c->stream, &g);
     if (rc) return rc;
     if (e != hipSuccess) return plbfs_fail(PLBFS_ERR_HIP, "graph capture: %s", hipGetErrorString(e));
@@ -757,16 +812,25 @@ static int build_graph(plbfs_ctx *c) {
     return PLBFS_OK;
 }
 
+static void fill_result(plbfs_ctx *c, const Ctrl &h, float ms, plbfs_result *out) {
+    c->total_steps += h.step;
+    if (!out) return;
+    out->status = h.status;
+    out->steps = h.step;
+    out->converged = h.converged;
+    out->total_steps = c->total_steps;
+    for (int q = 0; q < 4; q++) out->residual[q] = h.res[q];
+    out->solve_ms = ms;
+}
+
 extern "C" int plbfs_run(plbfs_ctx *c, int32_t max_steps, plbfs_result *out) {
     if (!c || max_steps < 0) return plbfs_fail(PLBFS_ERR_INPUT, "bad arguments");
     int rc = ensure_hist(c, max_steps);
     if (rc) return rc;
     Ctrl h{};
     HIP_TRY(hipMemcpyAsync(c->ctrl, &h, sizeof(Ctrl), hipMemcpyHostToDevice, c->stream));
-    if (c->f32) launch_init_dt<float>(c, c->dm32);
-    else launch_init_dt<double>(c, c->dm);
-    launch_dt_modes(c);
-    HIP_TRY(hipGetLastError());
+    const RunScratch w = ctx_scratch(c);
+    if ((rc = launch_run_start(c, w))) return rc;
     if ((rc = build_graph(c))) return rc;
     const int interval = std::max(1, c->params.check_interval);
     hipEvent_t ev0, ev1;
@@ -776,7 +840,7 @@ extern "C" int plbfs_run(plbfs_ctx *c, int32_t max_steps, plbfs_result *out) {
     for (int s = 0; s < max_steps; s++) {
         if (c->graph_ok && !c->timing) {
             HIP_TRY(hipGraphLaunch(c->graph_exec, c->stream));
-        } else if ((rc = launch_step(c))) {
+        } else if ((rc = launch_step(c, w, c->timing))) {
             return rc;
         }
         if ((s + 1) % interval == 0 || s == max_steps - 1) {
@@ -797,15 +861,61 @@ extern "C" int plbfs_run(plbfs_ctx *c, int32_t max_steps, plbfs_result *out) {
     if (h.step > 0)
         HIP_TRY(hipMemcpy(c->last_hist.data(), c->hist, (size_t)std::min(h.step, c->hist_cap) * 4 * sizeof(double),
                           hipMemcpyDeviceToHost));
-    c->total_steps += h.step;
-    if (out) {
-        out->status = h.status;
-        out->steps = h.step;
-        out->converged = h.converged;
-        out->total_steps = c->total_steps;
-        for (int q = 0; q < 4; q++) out->residual[q] = h.res[q];
-        out->solve_ms = ms;
+    fill_result(c, h, ms, out);
+    return h.status;
+}
+
+static size_t round256(size_t b) { return (b + 255) / 256 * 256; }
+
+// Bytes of scratch a run of max_steps steps needs on context c.
+static size_t scratch_need(const plbfs_ctx *c, int64_t max_steps) {
+    return round256(sizeof(Ctrl)) + round256((size_t)c->L.n_blocks * 5 * sizeof(double)) +
+           round256((size_t)c->cell_grid * sizeof(double)) + (size_t)max_steps * 4 * sizeof(double);
+}
+
+__global__ void k_clear_ctrl(Ctrl *ctrl) {
+    if (threadIdx.x == 0) *ctrl = Ctrl{};
+}
+
+extern "C" int plbfs_run_noalloc(plbfs_ctx *c, int32_t max_steps, plbfs_result *out, double *hist_out) {
+    if (!c || max_steps < 0) return plbfs_fail(PLBFS_ERR_INPUT, "bad arguments");
+    if (!c->scratch) return plbfs_fail(PLBFS_ERR_HIP, "no scratch address for this context's device");
+    const size_t need = scratch_need(c, max_steps);
+    if (need > kScratchBytes)
+        return plbfs_fail(PLBFS_ERR_INPUT,
+                          "plbfs_run_noalloc: %d cells in %d blocks and %d steps need %zu B of scratch, limit %zu B",
+                          c->L.N, c->L.n_blocks, max_steps, need, kScratchBytes);
+    char *p = c->scratch;
+    RunScratch w;
+    w.ctrl = reinterpret_cast<Ctrl *>(p);
+    p += round256(sizeof(Ctrl));
+    w.partials = reinterpret_cast<double *>(p);
+    p += round256((size_t)c->L.n_blocks * 5 * sizeof(double));
+    w.bmin = reinterpret_cast<double *>(p);
+    p += round256((size_t)c->cell_grid * sizeof(double));
+    w.hist = reinterpret_cast<double *>(p);
+    w.hist_cap = max_steps;
+    const auto t0 = std::chrono::steady_clock::now();
+    hipLaunchKernelGGL(k_clear_ctrl, dim3(1), dim3(64), 0, c->stream, w.ctrl);
+    int rc = launch_run_start(c, w);
+    if (rc) return rc;
+    const int interval = std::max(1, c->params.check_interval);
+    Ctrl h{};
+    for (int s = 0; s < max_steps; s++) {
+        if ((rc = launch_step(c, w, false))) return rc;
+        if ((s + 1) % interval == 0 || s == max_steps - 1) {
+            HIP_TRY(hipMemcpyAsync(&h, w.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, c->stream));
+            HIP_TRY(hipStreamSynchronize(c->stream));
+            if (h.done) break;
+        }
     }
+    HIP_TRY(hipMemcpyAsync(&h, w.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (hist_out && h.step > 0)
+        HIP_TRY(hipMemcpy(hist_out, w.hist, (size_t)h.step * 4 * sizeof(double), hipMemcpyDeviceToHost));
+    c->last_hist.clear();  // plbfs_get_residual_history covers plbfs_run only
+    fill_result(c, h, (float)ms, out);
     return h.status;
 }
 
@@ -871,10 +981,8 @@ extern "C" int plbfs_get_dt(plbfs_ctx *c, double *dt) {
     if (!c || !dt) return plbfs_fail(PLBFS_ERR_INPUT, "null argument");
     Ctrl h{};
     HIP_TRY(hipMemcpyAsync(c->ctrl, &h, sizeof(Ctrl), hipMemcpyHostToDevice, c->stream));
-    if (c->f32) launch_init_dt<float>(c, c->dm32);
-    else launch_init_dt<double>(c, c->dm);
-    launch_dt_modes(c);
-    HIP_TRY(hipGetLastError());
+    int rc = launch_run_start(c, ctx_scratch(c));
+    if (rc) return rc;
     HIP_TRY(hipStreamSynchronize(c->stream));
     double *outs[1] = {dt};
     return from_device_fields(c, c->dtl, 1, outs);

@@ -24,7 +24,7 @@ DELTA_HAT2, DELTA_PESKIN4 = 0, 1
 EXPORTS = [
     "plbfs_last_error", "plbfs_device_count", "plbfs_create", "plbfs_destroy", "plbfs_set_state",
     "plbfs_get_state", "plbfs_set_ibm", "plbfs_get_ibm", "plbfs_get_body_force", "plbfs_run",
-    "plbfs_get_residual_history", "plbfs_eval_residual", "plbfs_get_dt", "plbfs_enable_timing",
+    "plbfs_run_noalloc", "plbfs_get_residual_history", "plbfs_eval_residual", "plbfs_get_dt", "plbfs_enable_timing",
     "plbfs_kernel_stats", "plbfs_layout_info", "plbfs_wall_forces", "plbfs_ibm_body_force",
     "plbfs_mesh_read_openfoam", "plbfs_mesh_from_arrays", "plbfs_mesh_free", "plbfs_mesh_info",
     "plbfs_mesh_view_get", "plbfs_mesh_bface_patch", "plbfs_mesh_patch_type", "plbfs_mesh_raw",
@@ -101,6 +101,7 @@ def lib():
     L.plbfs_get_ibm.argtypes = [vp, dp, dp, dp]
     L.plbfs_get_body_force.argtypes = [vp, dp]
     L.plbfs_run.argtypes = [vp, C.c_int32, C.POINTER(Result)]
+    L.plbfs_run_noalloc.argtypes = [vp, C.c_int32, C.POINTER(Result), dp]
     L.plbfs_get_residual_history.argtypes = [vp, dp, C.c_int32]
     L.plbfs_eval_residual.argtypes = [vp, dp, dp]
     L.plbfs_get_dt.argtypes = [vp, dp]

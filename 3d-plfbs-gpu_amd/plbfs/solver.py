@@ -178,6 +178,17 @@ class GpuSolver:
         return dict(rc=rc, status=r.status, steps=r.steps, converged=bool(r.converged), total=r.total_steps,
                     residual=list(r.residual), ms=r.solve_ms, hist=hist[: r.steps])
 
+    def run_noalloc(self, nsteps):
+        """plbfs_run_noalloc: the same steps as run(), working in the library's
+        fixed per-device scratch array instead of allocated memory."""
+        r = nat.Result()
+        hist = np.zeros((max(int(nsteps), 1), 4))
+        rc = nat.lib().plbfs_run_noalloc(self.h, int(nsteps), C.byref(r), nat.dptr(hist))
+        if rc not in (0, -2, -3, -7):
+            nat.check(rc)
+        return dict(rc=rc, status=r.status, steps=r.steps, converged=bool(r.converged), total=r.total_steps,
+                    residual=list(r.residual), ms=r.solve_ms, hist=hist[: r.steps])
+
     def eval_residual(self):
         g = np.zeros((self.N, 12))
         R = np.zeros((self.N, 4))

@@ -166,6 +166,18 @@ int plbfs_get_body_force(plbfs_ctx *ctx, double *f);
 /* Advances up to max_steps time steps (4 RK stages each), stopping after the
  * first step whose residual is below params.tolerance (src/Solver.cpp:522). */
 int plbfs_run(plbfs_ctx *ctx, int32_t max_steps, plbfs_result *out);
+/* plbfs_run without allocation.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  It runs the same
+ * kernels with the same results as plbfs_run, but launches them directly (no
+ * graph, no events; solve_ms is host wall time) and keeps the control block,
+ * per-block residual partials, dt minima and the residual history in a
+ * fixed array of 8 MiB per device, held until the process ends.  A call needs
+ * about 256 B + 40 B per flux block + 8 B per 256 cells + 32 B per step of it
+ * and fails with PLBFS_ERR_INPUT beyond that.  hist_out [4*max_steps] (or
+ * NULL) receives the residual history; plbfs_get_residual_history keeps to
+ * plbfs_run.  Calls that overlap on one device share the array and race. */
+int plbfs_run_noalloc(plbfs_ctx *ctx, int32_t max_steps, plbfs_result *out, double *hist_out);
 /* Per-step residual history of the last plbfs_run call, [4*steps]. */
 int plbfs_get_residual_history(plbfs_ctx *ctx, double *out, int32_t max_steps);
 
