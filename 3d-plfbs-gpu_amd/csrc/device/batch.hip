@@ -608,6 +608,11 @@ extern "C" int plbfs_batch_create(const plbfs_mesh_view *mesh, const plbfs_param
         return plbfs_fail(PLBFS_ERR_INPUT, "batched cases support local and constant time stepping");
     for (int c = 0; c < n_cases; c++)
         if (!(case_visc[c] > 0)) return plbfs_fail(PLBFS_ERR_INPUT, "case %d: viscosity must be positive", c);
+    for (int q = 0; q < n_patches; q++)
+        if (rho_keeps_ghost(patches[q].rho_type) || vel_keeps_ghost(patches[q].vel_type))
+            return plbfs_fail(PLBFS_ERR_BCTYPE,
+                              "patch %d: types that keep the ghost's initial state (density 4, 5, 9, velocity "
+                              "4, 5, 7, 9) need a per-case ghost state, which batches do not hold", q);
     auto *b = new plbfs_batch();
     const int N = mesh->n_cells, F = mesh->n_faces, NI = mesh->n_internal_faces, T = N + F - NI;
     b->N = N;

@@ -63,6 +63,9 @@ struct plbfs_ctx {
     double *grad = nullptr;                 // 12 per cell
     double *dtl = nullptr;                  // [N]
     double *fbody = nullptr;                // [3][N] immersed-boundary force density
+    void *b_fix = nullptr;                  // [G] fixed ghost states (storage precision)
+    std::vector<double> ghost_fixed;        // [4*G] the same in FP64 on the host
+    bool ghost_state_set = true;            // false until plbfs_set_ghost_state where needed
     void *bflux = nullptr;                  // [4][n_border] border-face fluxes (storage precision)
     double *partials = nullptr;             // [n_blocks*5]
     double *bmin = nullptr;                 // [n_blocks]

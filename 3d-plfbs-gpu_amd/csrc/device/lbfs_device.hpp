@@ -17,8 +17,14 @@
 
 namespace plbfs {
 
+// flags: the ghost gradient copies cell b_partner[b]'s (periodic partner, or
+// cell 0 where the reference's update leaves the ghost unwritten), and the
+// ghost state holds per-ghost fixed values b_fix[b] (those patches)
+constexpr int32_t kGhostGradFromCell = 1;
+constexpr int32_t kGhostFixed = 2;
+
 struct DevPatch {
-    int32_t rho_type, vel_type, is_empty, pad;
+    int32_t rho_type, vel_type, is_empty, flags;
     double rho, u, v, w;
 };
 
@@ -104,7 +110,8 @@ struct DevMesh {
     // spreads force to; null: every block reads the body force
     const int32_t *blk_force;
     const int32_t *b_patch;
-    const int32_t *b_partner;  // periodic partner cell of each boundary face, -1 elsewhere
+    const int32_t *b_partner;  // cell whose gradient the ghost takes (kGhostGradFromCell), -1 elsewhere
+    const V4<S> *b_fix;        // [G] fixed ghost states (kGhostFixed); FP32: rho - rho0
     const S *b_dmag;
     const DevPatch *patches;  // FP32 storage: rho values shifted by -rho0
     const S *corr;            // FP32 storage: [3][N] rho0/3 sum(+-n A)/V per cell
@@ -121,15 +128,23 @@ struct DevParams {
 // Ghost cells (src/Solution.cpp:759-833, src/gradients.cpp:395-504,592-648)
 // ---------------------------------------------------------------------------
 
-// pp: state of the periodic partner (read only by type 3).
+// pp: state of the periodic partner (read only by type 3); fx: the ghost's
+// fixed values (read only by the types the reference's update leaves at the
+// ghost's initial state, and the parabolic u, v of types 4 and 5).
 template <typename T>
-__device__ __forceinline__ void ghost_state(const DevPatch &P, const T po[4], const T pp[4], T pg[4]) {
+__device__ __forceinline__ void ghost_state(const DevPatch &P, const T po[4], const T pp[4], const T fx[4],
+                                            T pg[4]) {
     const int rt = P.rho_type, vt = P.vel_type;
     if (rt == 1) pg[0] = (T)P.rho - (po[0] - (T)P.rho);
     else if (rt == 3) pg[0] = pp[0];
     else if (rt == 6 || rt == 7) pg[0] = (T)P.rho;
+    else if (rt == 4 || rt == 5 || rt == 9) pg[0] = fx[0];
     else pg[0] = po[0];  // 2 (neumann, dx = 0) and 8 (symmetry)
-    if (vt == 1) {
+    if (vt == 4 || vt == 5 || vt == 7 || vt == 9) {
+        pg[1] = fx[1];
+        pg[2] = fx[2];
+        pg[3] = fx[3];
+    } else if (vt == 1) {
         pg[1] = (T)P.u - (po[1] - (T)P.u);
         pg[2] = (T)P.v - (po[2] - (T)P.v);
         pg[3] = (T)P.w - (po[3] - (T)P.w);
@@ -152,9 +167,10 @@ __device__ __forceinline__ void ghost_state(const DevPatch &P, const T po[4], co
     }
 }
 
-// Ghost gradient from the owner's state and gradient (gp: the periodic
-// partner's gradient, read only by type 3).  rho_type 8 leaves the
-// reference's zero-initialised rho slot untouched, so it is zero here too.
+// Ghost gradient from the owner's state and gradient (gp: the gradient of
+// cell b_partner[b], read by type 3 and by the types whose ghost the
+// reference's update leaves unwritten).  rho_type 8 leaves the reference's
+// zero-initialised rho slot untouched, so it is zero here too.
 template <typename T>
 __device__ __forceinline__ void ghost_grad(const DevPatch &P, const T n[3], T dmag, const T po[4], const T go[12],
                                            const T gp[12], T gg[12]) {
@@ -172,7 +188,7 @@ __device__ __forceinline__ void ghost_grad(const DevPatch &P, const T n[3], T dm
         gg[0] = go[0];
         gg[1] = go[1];
         gg[2] = go[2];
-    } else if (rt == 3) {
+    } else if (rt == 3 || rt == 4 || rt == 5 || rt == 9) {
         gg[0] = gp[0];
         gg[1] = gp[1];
         gg[2] = gp[2];
@@ -196,7 +212,7 @@ __device__ __forceinline__ void ghost_grad(const DevPatch &P, const T n[3], T dm
             gg[6 + c] = go[6 + c] * ((T)1 - n[c]);
             gg[9 + c] = go[9 + c] * ((T)1 - n[c]);
         }
-    } else if (vt == 3) {
+    } else if (vt == 3 || vt == 4 || vt == 5 || vt == 7 || vt == 9) {
 #pragma unroll
         for (int q = 3; q < 12; q++) gg[q] = gp[q];
     } else {

@@ -42,7 +42,8 @@ extern "C" int plbfs_wall_forces(plbfs_ctx *c, const plbfs_mesh_view *mesh, cons
                 gu[a] = g[(size_t)12 * o + 3 + a] * (1 - n[a]);
                 gv[a] = g[(size_t)12 * o + 6 + a] * (1 - n[a]);
             }
-        } else if (P.vel_type == 3) {
+        } else if (P.vel_type == 3 || plbfs::vel_keeps_ghost(P.vel_type)) {
+            // the periodic partner's gradient; cell 0's where the ghost is not written
             const int p = c->L.perm[c->L.b_partner[b]];
             for (int a = 0; a < 3; a++) {
                 gu[a] = g[(size_t)12 * p + 3 + a];

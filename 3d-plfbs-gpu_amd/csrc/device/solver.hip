@@ -423,6 +423,22 @@ extern "C" int plbfs_create(const plbfs_mesh_view *mesh, const plbfs_params *par
     return create_ctx(mesh, params, patches, n_patches, bface_patch, block_cells, 1, flux_stage, split, out);
 }
 
+// Fixed ghost states to the device: (rho, u, v, w) per boundary face in the
+// storage precision (FP32: rho as the deviation from rho0).
+static int upload_ghost_fixed(plbfs_ctx *c) {
+    const size_t G = (size_t)c->L.G;
+    if (G == 0) return PLBFS_OK;
+    if (!c->f32) {
+        HIP_TRY(hipMemcpy(c->b_fix, c->ghost_fixed.data(), G * 4 * sizeof(double), hipMemcpyHostToDevice));
+    } else {
+        std::vector<float> h(G * 4);
+        for (size_t k = 0; k < G * 4; k++)
+            h[k] = (float)(c->ghost_fixed[k] - ((k & 3) == 0 ? c->prm.rho0 : 0.0));
+        HIP_TRY(hipMemcpy(c->b_fix, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
+    }
+    return PLBFS_OK;
+}
+
 int create_ctx(const plbfs_mesh_view *mesh, const plbfs_params *params, const plbfs_patch *patches,
                int32_t n_patches, const int32_t *bface_patch, int block_cells, int lds_cases, bool flux_stage,
                bool split_border, plbfs_ctx **out) {
@@ -493,7 +509,9 @@ int create_ctx(const plbfs_mesh_view *mesh, const plbfs_params *params, const pl
         dp[q].rho_type = patches[q].rho_type;
         dp[q].vel_type = patches[q].vel_type;
         dp[q].is_empty = patches[q].is_empty;
-        dp[q].pad = 0;
+        const bool keeps = rho_keeps_ghost(patches[q].rho_type) || vel_keeps_ghost(patches[q].vel_type);
+        dp[q].flags = (patches[q].rho_type == 3 || patches[q].vel_type == 3 || keeps ? kGhostGradFromCell : 0) |
+                      (keeps ? kGhostFixed : 0);
         dp[q].rho = patches[q].rho;
         dp[q].u = patches[q].u;
         dp[q].v = patches[q].v;
@@ -505,6 +523,10 @@ int create_ctx(const plbfs_mesh_view *mesh, const plbfs_params *params, const pl
     if (bpartner.empty()) bpartner.push_back(-1);
     std::vector<double> bdmag = L.b_dmag;
     if (bdmag.empty()) bdmag.push_back(1.0);
+    c->ghost_fixed = L.b_fixed;
+    c->ghost_state_set = !L.needs_ghost_state;
+    if ((rc = dalloc(c, (void **)&c->b_fix, (size_t)std::max(L.G, 1) * 4 * sbytes))) return fail(rc);
+    d.b_fix = c->f32 ? nullptr : (const double4 *)c->b_fix;
     d.corr = nullptr;
     if ((rc = upload(c, (uint8_t **)&d.st_cnt, L.st_cnt)) || (rc = upload(c, (int32_t **)&d.st_nbr, L.st_nbr)) ||
         (rc = upload(c, (double **)&d.vol, L.vol)) || (rc = upload(c, (double **)&d.inv_vol, L.inv_vol)) ||
@@ -549,6 +571,7 @@ int create_ctx(const plbfs_mesh_view *mesh, const plbfs_params *params, const pl
         e.st_cnt = d.st_cnt; e.st_nbr = d.st_nbr; e.vol = d.vol; e.inv_vol = d.inv_vol; e.cfl_area = d.cfl_area;
         e.blk_item_off = d.blk_item_off; e.it_x = nullptr; e.g_off = d.g_off;
         e.g_ent = d.g_ent; e.b_patch = d.b_patch; e.b_partner = d.b_partner;
+        e.b_fix = (const float4 *)c->b_fix;
         auto to32 = [](const std::vector<double> &v) { return std::vector<float>(v.begin(), v.end()); };
         std::vector<DevPatch> dp32 = dp;
         for (auto &q : dp32) q.rho -= p.ref_rho;
@@ -610,6 +633,7 @@ int create_ctx(const plbfs_mesh_view *mesh, const plbfs_params *params, const pl
     prm.use_body_force = 0;
     const char *g = std::getenv("PLBFS_NO_GRAPH");
     c->use_graph = !(g && g[0] == '1');
+    if ((rc = upload_ghost_fixed(c))) return fail(rc);
     *out = c;
     return PLBFS_OK;
 }
@@ -662,6 +686,37 @@ extern "C" int plbfs_set_state(plbfs_ctx *c, const double *rho, const double *u,
     }
     c->total_steps = 0;
     return PLBFS_OK;
+}
+
+extern "C" int plbfs_set_ghost_state(plbfs_ctx *c, const double *rho, const double *u, const double *v,
+                                     const double *w) {
+    if (!c || !rho || !u || !v || !w) return plbfs_fail(PLBFS_ERR_INPUT, "null argument");
+    // only the components the reference's ghost update leaves unwritten take
+    // these values (src/Solution.cpp:759-833); the parabolic u, v stay
+    for (int b = 0; b < c->L.G; b++) {
+        const plbfs_patch &P = c->patches[c->L.b_patch[b]];
+        double *q = &c->ghost_fixed[(size_t)4 * b];
+        if (rho_keeps_ghost(P.rho_type)) q[0] = rho[b];
+        if (P.vel_type == 7 || P.vel_type == 9) {
+            q[1] = u[b];
+            q[2] = v[b];
+        }
+        if (vel_keeps_ghost(P.vel_type)) q[3] = w[b];
+    }
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    int rc = upload_ghost_fixed(c);
+    if (rc) return rc;
+    c->ghost_state_set = true;
+    return PLBFS_OK;
+}
+
+// Refuses to advance a context whose patches keep initial ghost values that
+// were never given.
+static int check_ghost_state(const plbfs_ctx *c) {
+    if (c->ghost_state_set) return PLBFS_OK;
+    return plbfs_fail(PLBFS_ERR_INPUT,
+                      "a patch keeps the ghost's initial state (density 4, 5, 9 or velocity 4, 5, 7, 9): "
+                      "call plbfs_set_ghost_state first");
 }
 
 extern "C" int plbfs_get_state(plbfs_ctx *c, double *rho, double *u, double *v, double *w) {
@@ -825,7 +880,9 @@ static void fill_result(plbfs_ctx *c, const Ctrl &h, float ms, plbfs_result *out
 
 extern "C" int plbfs_run(plbfs_ctx *c, int32_t max_steps, plbfs_result *out) {
     if (!c || max_steps < 0) return plbfs_fail(PLBFS_ERR_INPUT, "bad arguments");
-    int rc = ensure_hist(c, max_steps);
+    int rc = check_ghost_state(c);
+    if (rc) return rc;
+    if ((rc = ensure_hist(c, max_steps))) return rc;
     if (rc) return rc;
     Ctrl h{};
     HIP_TRY(hipMemcpyAsync(c->ctrl, &h, sizeof(Ctrl), hipMemcpyHostToDevice, c->stream));
@@ -880,6 +937,7 @@ __global__ void k_clear_ctrl(Ctrl *ctrl) {
 extern "C" int plbfs_run_noalloc(plbfs_ctx *c, int32_t max_steps, plbfs_result *out, double *hist_out) {
     if (!c || max_steps < 0) return plbfs_fail(PLBFS_ERR_INPUT, "bad arguments");
     if (!c->scratch) return plbfs_fail(PLBFS_ERR_HIP, "no scratch address for this context's device");
+    if (int e = check_ghost_state(c)) return e;
     const size_t need = scratch_need(c, max_steps);
     if (need > kScratchBytes)
         return plbfs_fail(PLBFS_ERR_INPUT,
@@ -928,6 +986,7 @@ extern "C" int plbfs_get_residual_history(plbfs_ctx *c, double *out, int32_t max
 
 extern "C" int plbfs_eval_residual(plbfs_ctx *c, double *grads, double *R) {
     if (!c) return plbfs_fail(PLBFS_ERR_INPUT, "null argument");
+    if (int e = check_ghost_state(c)) return e;
     const int N = c->L.N, nb = c->L.n_blocks;
     Ctrl h{};
     HIP_TRY(hipMemcpyAsync(c->ctrl, &h, sizeof(Ctrl), hipMemcpyHostToDevice, c->stream));

@@ -111,8 +111,14 @@ __device__ __forceinline__ void lsq_grad(const DevMesh<S> &m, int i, const ST &s
                 pj[3] = qj.w;
             } else {
                 // (periodic ghosts were replaced by their partner cell in st_nbr)
-                const DevPatch P = m.patches[m.b_patch[-1 - j]];
-                ghost_state<S>(P, p0, p0, pj);
+                const int b = -1 - j;
+                const DevPatch P = m.patches[m.b_patch[b]];
+                S fx[4] = {0, 0, 0, 0};
+                if (P.flags & kGhostFixed) {
+                    const V4<S> f = m.b_fix[b];
+                    fx[0] = f.x; fx[1] = f.y; fx[2] = f.z; fx[3] = f.w;
+                }
+                ghost_state<S>(P, p0, p0, fx, pj);
             }
             const S wx = m.st_w[((size_t)s * 3 + 0) * N + i];
             const S wy = m.st_w[((size_t)s * 3 + 1) * N + i];
@@ -276,9 +282,13 @@ __device__ __forceinline__ void item_flux(const DevMesh<S> &m, const DevParams &
             ld.side(nb, pn, gN);
         } else {
             const DevPatch P = m.patches[m.b_patch[0]];
-            S pp[4], gp[12];
-            if (P.rho_type == 3 || P.vel_type == 3) ld.side(m.b_partner[0], pp, gp);
-            ghost_state<S>(P, po, pp, pn);
+            S pp[4], gp[12], fx[4] = {0, 0, 0, 0};
+            if (P.flags & kGhostGradFromCell) ld.side(m.b_partner[0], pp, gp);
+            if (P.flags & kGhostFixed) {
+                const V4<S> f = m.b_fix[0];
+                fx[0] = f.x; fx[1] = f.y; fx[2] = f.z; fx[3] = f.w;
+            }
+            ghost_state<S>(P, po, pp, fx, pn);
             ghost_grad<S>(P, n, m.b_dmag[0], po, go, gp, gN);
         }
 #pragma unroll
@@ -291,7 +301,7 @@ __device__ __forceinline__ void item_flux(const DevMesh<S> &m, const DevParams &
         const int b = -1 - nb;
         const DevPatch P = m.patches[m.b_patch[b]];
         S gp[12];
-        if (P.rho_type == 3 || P.vel_type == 3) {
+        if (P.flags & kGhostGradFromCell) {
             S pp[4];
             ld.side(m.b_partner[b], pp, gp);
         }

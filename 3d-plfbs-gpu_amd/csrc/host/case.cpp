@@ -196,6 +196,9 @@ static int case_load(const char *xml_path, plbfs_case **out) {
     p.precision = c->s("global_variables/precision") == "fp32" ? PLBFS_FP32 : PLBFS_FP64;
     p.check_interval = (int32_t)c->n("global_variables/check_interval", 10);
     p.lsq_exact_det = c->s("global_variables/lsq_determinant") == "exact" ? 1 : 0;
+    // domain.X, domain.Y after domain_geometry::scale_geometries (domain_geometry.cpp:21-24)
+    p.domain_x = c->n("geometry/x", 1.0) * c->n("global_variables/scale", 1.0);
+    p.domain_y = c->n("geometry/y", 1.0) * c->n("global_variables/scale", 1.0);
     if (!std::isfinite(p.visc))
         return plbfs_fail(PLBFS_ERR_INPUT, "%s: mach_no * ref_length / reynolds_no is not finite", xml_path);
     *out = c.release();
@@ -487,10 +490,10 @@ static int program_run(const char *xml_path, plbfs_result *result) {
     std::vector<double> rho(T), u(T), v(T), w(T);
     std::string outdir = c->s("global_variables/output_directory", ".");
     mkdirs(outdir);
+    // initial conditions over all cells; a restart replaces the interior ones
+    if ((rc = plbfs_case_initial_state(c, m, rho.data(), u.data(), v.data(), w.data()))) return rc;
     if (c->s("global_variables/restart_analysis") == "true") {
         if ((rc = plbfs_read_fields(outdir.c_str(), N, rho.data(), u.data(), v.data(), w.data()))) return rc;
-    } else if ((rc = plbfs_case_initial_state(c, m, rho.data(), u.data(), v.data(), w.data()))) {
-        return rc;
     }
     plbfs_mesh_view view;
     plbfs_mesh_view_get(m, &view);
@@ -498,6 +501,9 @@ static int program_run(const char *xml_path, plbfs_result *result) {
     if ((rc = plbfs_create(&view, &p, patches.data(), NP, plbfs_mesh_bface_patch(m), &ctx))) return rc;
     std::unique_ptr<plbfs_ctx, int (*)(plbfs_ctx *)> xg(ctx, plbfs_destroy);
     if ((rc = plbfs_set_state(ctx, rho.data(), u.data(), v.data(), w.data()))) return rc;
+    // the initial state over the ghosts (soln's entries past n_cells), which
+    // some boundary types keep (src/Solution.cpp:759-833)
+    if ((rc = plbfs_set_ghost_state(ctx, rho.data() + N, u.data() + N, v.data() + N, w.data() + N))) return rc;
     if (!c->objects.empty()) {
         std::vector<double> all;
         double stiff = 0, area = 0;

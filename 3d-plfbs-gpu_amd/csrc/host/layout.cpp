@@ -47,8 +47,8 @@ uint64_t hilbert_key3(uint32_t x, uint32_t y, uint32_t z, int bits) {
     return key;
 }
 
-static bool rho_type_ok(int t) { return t == 1 || t == 2 || t == 3 || t == 6 || t == 7 || t == 8; }
-static bool vel_type_ok(int t) { return t == 1 || t == 2 || t == 3 || t == 6 || t == 8; }
+// the type codes of src/preprocessor.cpp:239-258
+static bool type_ok(int t) { return t >= 1 && t <= 9; }
 
 // Interior cell nearest to x (file index; ties to the smaller index), found
 // through a uniform bin grid of the interior centroids.
@@ -124,7 +124,7 @@ int build_layout(const plbfs_mesh_view &m, const plbfs_params &p, const plbfs_pa
         return plbfs_fail(PLBFS_ERR_INPUT, "incomplete mesh view");
     if (n_patches <= 0 || !patches || (G > 0 && !bface_patch)) return plbfs_fail(PLBFS_ERR_INPUT, "missing patches");
     for (int q = 0; q < n_patches; q++) {
-        if (!rho_type_ok(patches[q].rho_type) || !vel_type_ok(patches[q].vel_type))
+        if (!type_ok(patches[q].rho_type) || !type_ok(patches[q].vel_type))
             return plbfs_fail(PLBFS_ERR_BCTYPE, "patch %d: unsupported bc types rho=%d vel=%d", q,
                               patches[q].rho_type, patches[q].vel_type);
         if ((patches[q].rho_type == 3) != (patches[q].vel_type == 3))
@@ -357,6 +357,30 @@ int build_layout(const plbfs_mesh_view &m, const plbfs_params &p, const plbfs_pa
                 const int j = L.st_nbr[k];
                 if (j < 0 && L.b_partner[-1 - j] >= 0) L.st_nbr[k] = L.b_partner[-1 - j];
             }
+        }
+    }
+
+    // ghosts the reference leaves (partly) unwritten: fixed values, and the
+    // gradient of cell 0, its periodic node on a patch that is not periodic
+    // (src/Boundary_Conditions.cpp:416); parabolic u, v at the centroid of
+    // cell b, b the boundary index (src/Solution.cpp:804-813)
+    L.b_fixed.assign((size_t)4 * G, 0.0);
+    L.needs_ghost_state = false;
+    for (int b = 0; b < G; b++) {
+        const plbfs_patch &P = patches[bface_patch[b]];
+        const bool kr = rho_keeps_ghost(P.rho_type), kv = vel_keeps_ghost(P.vel_type);
+        if (!kr && !kv) continue;
+        L.needs_ghost_state = true;
+        L.b_partner[b] = L.inv[0];
+        const double xb = cc[3 * b], yb = cc[3 * b + 1];
+        if (P.vel_type == 4) {
+            const double Y = p.domain_y;
+            L.b_fixed[(size_t)4 * b + 1] = 4 * P.u / std::pow(Y, 2) * yb * (Y - yb);
+            L.b_fixed[(size_t)4 * b + 2] = 4 * P.v / std::pow(Y, 2) * yb * (Y - yb);
+        } else if (P.vel_type == 5) {
+            const double X = p.domain_x;
+            L.b_fixed[(size_t)4 * b + 1] = 4 * P.u / std::pow(X, 2) * xb * (X - yb);
+            L.b_fixed[(size_t)4 * b + 2] = 4 * P.v / std::pow(X, 2) * xb * (X - xb);
         }
     }
 

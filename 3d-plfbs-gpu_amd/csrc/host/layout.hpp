@@ -66,11 +66,26 @@ struct Layout {
     std::vector<int32_t> b_patch;  // [G]
     std::vector<double> b_dmag;    // [G] |x_ghost - x_owner|
     std::vector<int32_t> b_owner;  // [G] new owner index
-    std::vector<int32_t> b_partner;  // [G] periodic partner (new index), -1 elsewhere
+    // [G] cell whose gradient the ghost takes (new index): the periodic partner
+    // on periodic patches, cell 0 (file order) on patches whose ghost the
+    // reference leaves unwritten (density 4, 5, 9; velocity 4, 5, 7, 9);
+    // -1 elsewhere
+    std::vector<int32_t> b_partner;
+    // [4*G] (rho, u, v, w) of ghosts that keep fixed values: u, v of the
+    // parabolic profiles (velocity 4, 5) here; the components the reference
+    // leaves at the ghost's initial state come from plbfs_set_ghost_state
+    std::vector<double> b_fixed;
+    bool needs_ghost_state = false;  // some patch keeps initial ghost values
     // LSQ inverse and RHS in the reference layout, file order (for tests)
     std::vector<double> lsq_inv9;  // [N*9]
     std::vector<double> lsq_rhs;   // [N*6*3]
 };
+
+// Boundary types whose ghost the reference's update leaves (partly) at its
+// initial value, and whose ghost gradient is then the periodic node's: cell 0
+// (src/Solution.cpp:759-833, src/gradients.cpp:434-440,488-502).
+inline bool rho_keeps_ghost(int t) { return t == 4 || t == 5 || t == 9; }
+inline bool vel_keeps_ghost(int t) { return t == 4 || t == 5 || t == 7 || t == 9; }
 
 // Builds the layout.  Returns PLBFS_OK or an error code (message via plbfs_fail).
 int build_layout(const plbfs_mesh_view &m, const plbfs_params &p, const plbfs_patch *patches, int n_patches,

@@ -23,7 +23,7 @@ DELTA_HAT2, DELTA_PESKIN4 = 0, 1
 # every symbol the two headers declare
 EXPORTS = [
     "plbfs_last_error", "plbfs_device_count", "plbfs_create", "plbfs_destroy", "plbfs_set_state",
-    "plbfs_get_state", "plbfs_set_ibm", "plbfs_get_ibm", "plbfs_get_body_force", "plbfs_run",
+    "plbfs_set_ghost_state", "plbfs_get_state", "plbfs_set_ibm", "plbfs_get_ibm", "plbfs_get_body_force", "plbfs_run",
     "plbfs_run_noalloc", "plbfs_get_residual_history", "plbfs_eval_residual", "plbfs_get_dt", "plbfs_enable_timing",
     "plbfs_kernel_stats", "plbfs_layout_info", "plbfs_wall_forces", "plbfs_ibm_body_force",
     "plbfs_mesh_read_openfoam", "plbfs_mesh_from_arrays", "plbfs_mesh_free", "plbfs_mesh_info",
@@ -58,7 +58,8 @@ class Params(C.Structure):
     _fields_ = [("visc", C.c_double), ("gamma", C.c_double), ("fneq_min", C.c_double), ("cfl", C.c_double),
                 ("max_velocity", C.c_double), ("ref_rho", C.c_double), ("force", C.c_double),
                 ("tolerance", C.c_double), ("gradient_type", C.c_int32), ("time_stepping", C.c_int32),
-                ("precision", C.c_int32), ("check_interval", C.c_int32), ("lsq_exact_det", C.c_int32)]
+                ("precision", C.c_int32), ("check_interval", C.c_int32), ("lsq_exact_det", C.c_int32),
+                ("domain_x", C.c_double), ("domain_y", C.c_double)]
 
 
 class IbmNodes(C.Structure):
@@ -97,6 +98,7 @@ def lib():
     L.plbfs_destroy.argtypes = [vp]
     L.plbfs_set_state.argtypes = [vp, dp, dp, dp, dp]
     L.plbfs_get_state.argtypes = [vp, dp, dp, dp, dp]
+    L.plbfs_set_ghost_state.argtypes = [vp, dp, dp, dp, dp]
     L.plbfs_set_ibm.argtypes = [vp, C.POINTER(IbmNodes)]
     L.plbfs_get_ibm.argtypes = [vp, dp, dp, dp]
     L.plbfs_get_body_force.argtypes = [vp, dp]

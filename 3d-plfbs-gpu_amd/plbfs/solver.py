@@ -101,7 +101,8 @@ class HostMesh:
 def make_params(p):
     return nat.Params(p["visc"], p["gamma"], p["fneq_min"], p["cfl"], p["max_velocity"], p["ref_rho"], p["force"],
                       p["tolerance"], p["gradient_type"], p["time_stepping"], p.get("precision", 0),
-                      p.get("check_interval", 10), int(p.get("lsq_exact_det", 0)))
+                      p.get("check_interval", 10), int(p.get("lsq_exact_det", 0)), float(p.get("domain_x", 1.0)),
+                      float(p.get("domain_y", 1.0)))
 
 
 def make_patches(patches):
@@ -167,6 +168,11 @@ class GpuSolver:
         a = [np.zeros(self.N) for _ in range(4)]
         nat.check(nat.lib().plbfs_get_state(self.h, *(nat.dptr(x) for x in a)))
         return a
+
+    def set_ghost_state(self, rho, u, v, w):
+        """Initial ghost state, one value per boundary face (plbfs_set_ghost_state)."""
+        a = state_arrays((rho, u, v, w), self.mesh.n_faces - self.mesh.n_internal)
+        nat.check(nat.lib().plbfs_set_ghost_state(self.h, *(nat.dptr(x) for x in a)))
 
     def run(self, nsteps, raise_on_error=False):
         r = nat.Result()

@@ -76,8 +76,17 @@ typedef struct {
 } plbfs_mesh_view;
 
 /* One boundary patch (src/Boundary_Conditions.cpp:388-452): types use the
- * reference codes 1 dirichlet, 2 neumann, 3 periodic, 6 inlet, 7 wall (rho
- * only), 8 symmetry-x.  Velocities are already multiplied by mach_no/scale.
+ * reference codes of src/preprocessor.cpp:239-258, 1 dirichlet, 2 neumann,
+ * 3 periodic, 4 parabolic-N, 5 parabolic-W, 6 inlet, 7 wall, 8 symmetry-x,
+ * 9 parabolic, for density and velocity alike.  The ghost update
+ * (src/Solution.cpp:759-833) writes u and v of types 4 and 5 from a profile
+ * at the centroid of cell b, b the boundary index (domain_x, domain_y of
+ * plbfs_params), and writes nothing for density 4, 5, 9, velocity 7, 9 and w
+ * of 4, 5: those keep the ghost's initial state (plbfs_set_ghost_state).  The
+ * ghost gradient of all of them is the gradient of cell 0, the reference's
+ * periodic node of a patch that is not periodic (src/gradients.cpp:434-440,
+ * 488-502, src/Boundary_Conditions.cpp:416).  Velocities are already
+ * multiplied by mach_no/scale.
  * A periodic face's ghost takes the state and gradient of its partner: the
  * interior cell nearest to the owner's centroid + per_translate
  * (src/Boundary_Conditions.cpp:425-445, whose search starts from an
@@ -111,6 +120,9 @@ typedef struct {
                                    LHS_xy = 0 (axis-aligned hexes); on jittered
                                    tetrahedra the reference expression can flip the
                                    gradient's sign and the run diverges. */
+    double domain_x, domain_y;  /* domain.X, domain.Y: geometry x, y times scale
+                                   (src/preprocessor.cpp:48,467-468), the lengths
+                                   of the parabolic profiles of types 4 and 5 */
 } plbfs_params;
 
 /* Lagrangian nodes of the immersed objects (reference: lagrangian_object,
@@ -153,6 +165,12 @@ int plbfs_destroy(plbfs_ctx *ctx);
 /* Initial primitive state of interior cells (src/Solution.cpp:94-211); the
  * conservative accumulator starts as the same values (src/Solver.cpp:195). */
 int plbfs_set_state(plbfs_ctx *ctx, const double *rho, const double *u, const double *v, const double *w);
+/* Initial state of the ghost cells, [F - NI] each in boundary-face order (the
+ * reference's soln entries n_cells.., src/Solution.cpp:94-211 over all cells).
+ * The boundary types that leave a ghost component unwritten (density 4, 5, 9;
+ * velocity 7, 9; w of velocity 4, 5) keep these values; other ghosts ignore
+ * them.  A context with such a patch refuses to run until this was called. */
+int plbfs_set_ghost_state(plbfs_ctx *ctx, const double *rho, const double *u, const double *v, const double *w);
 /* Final primitive state (soln) of interior cells. */
 int plbfs_get_state(plbfs_ctx *ctx, double *rho, double *u, double *v, double *w);
 

@@ -90,3 +90,32 @@ def test_sphere_drag_points_downstream(gpu_available):
     _, _, fn = s.ibm_state()
     assert np.allclose(F, -fn.sum(0))
     assert F[0] > 0 and abs(F[1]) < 0.2 * F[0] and abs(F[2]) < 0.2 * F[0], F
+
+
+@pytest.mark.parametrize("tets", [False, True])
+def test_rigid_cylinder_coupled_steps_match_oracle(gpu_available, tets):
+    # the reference's only rigid object (lagrangian_object.cpp:79-113) with its
+    # own IBM parameters: 2-point hat of width delta_x = 2 dt (Solver_gpu.cu:517)
+    # and node area 2 pi r / n * depth (immersed_boundary_method.cu:168)
+    pm = channel_mesh(16, 10, 10, tets=tets, jitter=0.12)
+    case = pc.channel_case(re=40.0, length_scale=4.0)
+    case.lsq_exact_det = 1 if tets else 0
+    o = oracle_for(pm, case)
+    hm = HostMesh.from_polymesh(pm, case.visc, case.gamma, case.fneq_min)
+    s = GpuSolver(hm, case.params, case.patches_for(pm.patch_names))
+    st = perturbed_state(case, o, amp=0.01, seed=6)
+    o.set_state(*st)
+    s.set_state(*st)
+    X, dA = objects.rigid_cylinder(36, (6.0, 5.0, 5.0), 2.0, 6.0)
+    assert X.shape == (36, 3)
+    args = dict(stiffness=0.5, node_area=dA, dt=case.time_marching_step, h=2 * 0.5, kernel=0)
+    pyoracle.set_ibm(o, X, X, **args)
+    s.set_ibm(X, X, **args)
+    ro, rg = o.run(20), s.run(20)
+    assert ro["rc"] == 0 and rg["rc"] == 0, (ro["rc"], rg["rc"])
+    xo, uo, fo = pyoracle.ibm_state(o)
+    xg, ug, fg = s.ibm_state()
+    assert rel_err(xg - X, xo - X) < 1e-9 and rel_err(ug, uo) < 1e-9 and rel_err(fg, fo) < 1e-9
+    assert np.abs(xg - X).max() > 0  # the nodes moved
+    for a, b in zip(s.state(), o.state()):
+        assert rel_err(a, b) < 1e-10

@@ -489,3 +489,44 @@ def test_headers_are_plain_c(tmp_path):
     cells, faces, n_params, ok = r.stdout.split()
     assert (int(cells), int(faces), int(ok)) == (pm.n_cells, pm.n_faces, 1)
     assert int(n_params) > 0
+
+
+@pytest.mark.parametrize("num_nodes", [16, 20])
+def test_rigid_cylinder_object_nodes(tmp_path, num_nodes):
+    """A rigid_cylinder object (lagrangian_object.cpp:79-113): (int)sqrt(n)
+    rings of (int)sqrt(n) nodes, ring j at z = cz + depth/2 - j depth/rings -
+    depth/(2 rings), node i at angle 2 pi i/rings from +y towards +x (x = cx +
+    r sin, y = cy + r cos); node area 2 pi r / n * depth with the file's n
+    (immersed_boundary_method.cu:168).  With n = 20 the reference fills 16 of
+    its 20 node slots; the loader gives the 16 it fills."""
+    r, depth, cx, cy, cz = 2.0, 4.0, 5.0, 4.0, 3.0
+    obj = ("<lagrangian_objects><object><name>cyl</name><num_nodes>%d</num_nodes><radius>%g</radius>"
+           "<depth>%g</depth><stiffness>3</stiffness><centre_x>%g</centre_x><centre_y>%g</centre_y>"
+           "<centre_z>%g</centre_z><type>rigid_cylinder</type></object>"
+           "</lagrangian_objects>\n</root>" % (num_nodes, r, depth, cx, cy, cz))
+    xml = tmp_path / "case.xml"
+    xml.write_text(CASE_XML.format(out=str(tmp_path / "out"), mesh="unused/", prec="false").replace("</root>", obj))
+    L = nat.lib()
+    h = load_case(str(xml))
+    L.plbfs_case_object_nodes.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_double), C.c_int32,
+                                          C.POINTER(C.c_int32), C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                          C.POINTER(C.c_int32)]
+    x = np.zeros(3 * num_nodes)
+    n, k = C.c_int32(0), C.c_int32(0)
+    st, area = C.c_double(0), C.c_double(0)
+    nat.check(L.plbfs_case_object_nodes(h, 0, nat.dptr(x), num_nodes, C.byref(n), C.byref(st), C.byref(area),
+                                        C.byref(k)))
+    L.plbfs_case_free(h)
+    assert n.value == 16 and k.value == 0  # 4 rings of 4, the 2-point hat kernel
+    X = x[: 3 * 16].reshape(16, 3)
+    # closed form: rings at z = 3 + 2 - j - 0.5, quarter turns from +y
+    zs = [4.5, 3.5, 2.5, 1.5]
+    quarter = [(0.0, 1.0), (1.0, 0.0), (0.0, -1.0), (-1.0, 0.0)]
+    for j in range(4):
+        for i in range(4):
+            sx, sy = quarter[i]
+            assert np.allclose(X[4 * j + i], (cx + r * sx, cy + r * sy, zs[j]), rtol=0, atol=1e-14), (j, i)
+    assert area.value == pytest.approx(2 * np.pi * r / num_nodes * depth, rel=1e-15)
+    from plbfs import objects
+    Xp, dAp = objects.rigid_cylinder(num_nodes, (cx, cy, cz), r, depth)
+    assert np.array_equal(Xp, X) and dAp == area.value
