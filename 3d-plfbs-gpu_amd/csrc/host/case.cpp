@@ -424,11 +424,26 @@ extern "C" int plbfs_read_fields(const char *dir, int32_t n, double *rho, double
     return PLBFS_OK;
 }
 
-// ASCII Tecplot polyhedral zone with cell-centred variables (the content of
-// tecplot_output.cpp:273-466 and the grid writer unstructured_mesh.cpp:1990-2201,
-// without TecIO): left element = owner + 1, right = neighbour + 1 or 0.
-extern "C" int plbfs_write_tecplot(const plbfs_mesh *m, const char *path, const double *rho, const double *u,
-                                   const double *v, const double *w, double time) {
+// Tecplot ASCII output without TecIO.  Like the reference, the grid and the
+// solution are separate files: a grid file with the node coordinates and the
+// polyhedral face connectivity (unstructured_mesh.cpp:2073-2200; left element
+// = owner + 1, right = neighbour + 1 or 0), and per output time a solution
+// file with the cell-centred variables of tecplot_output.cpp:273-466.
+
+static void write_block(FILE *f, const std::vector<double> &a) {
+    const size_t n = a.size();
+    for (size_t i = 0; i < n; i++) std::fprintf(f, "%.17g%c", a[i], (i % 6 == 5 || i == n - 1) ? '\n' : ' ');
+}
+
+static void polyhedral_zone_header(FILE *f, int P, int N, int F, size_t nfv, double time) {
+    std::fprintf(f,
+                 "ZONE T=\"%d\" ZONETYPE=FEPOLYHEDRON NODES=%d ELEMENTS=%d FACES=%d TOTALNUMFACENODES=%zu "
+                 "NUMCONNECTEDBOUNDARYFACES=0 TOTALNUMBOUNDARYCONNECTIONS=0 DATAPACKING=BLOCK",
+                 N, P, N, F, nfv);
+    std::fprintf(f, " SOLUTIONTIME=%.17g", time);
+}
+
+extern "C" int plbfs_write_tecplot_grid(const plbfs_mesh *m, const char *path) {
     if (!m || !path) return plbfs_fail(PLBFS_ERR_INPUT, "null argument");
     int64_t info[6];
     plbfs_mesh_info(m, info);
@@ -438,23 +453,12 @@ extern "C" int plbfs_write_tecplot(const plbfs_mesh *m, const char *path, const 
     plbfs_mesh_raw(m, px.data(), py.data(), pz.data(), owner.data(), nb.data(), fo.data(), fv.data());
     FILE *f = std::fopen(path, "w");
     if (!f) return plbfs_fail(PLBFS_ERR_INPUT, "cannot write %s", path);
-    std::fprintf(f, "TITLE = \"PLBFS solution\"\nVARIABLES = \"X\" \"Y\" \"Z\" \"rho\" \"u\" \"v\" \"w\"\n");
-    std::fprintf(f,
-                 "ZONE T=\"t=%g\" ZONETYPE=FEPOLYHEDRON NODES=%d ELEMENTS=%d FACES=%d TOTALNUMFACENODES=%zu "
-                 "NUMCONNECTEDBOUNDARYFACES=0 TOTALNUMBOUNDARYCONNECTIONS=0 DATAPACKING=BLOCK "
-                 "VARLOCATION=([4-7]=CELLCENTERED) SOLUTIONTIME=%g\n",
-                 time, P, N, F, fv.size(), time);
-    auto col = [&](auto get, int n) {
-        for (int i = 0; i < n; i++) std::fprintf(f, "%.9g%c", (double)get(i), (i % 8 == 7 || i == n - 1) ? '\n' : ' ');
-    };
-    col([&](int i) { return px[i]; }, P);
-    col([&](int i) { return py[i]; }, P);
-    col([&](int i) { return pz[i]; }, P);
-    const double *fields[4] = {rho, u, v, w};
-    for (int q = 0; q < 4; q++) {
-        const double *a = fields[q];
-        col([&](int i) { return a ? a[i] : 0.0; }, N);
-    }
+    std::fprintf(f, "TITLE = \"LBFS grid\"\nFILETYPE = GRID\nVARIABLES = \"X\" \"Y\" \"Z\"\n");
+    polyhedral_zone_header(f, P, N, F, fv.size(), 0.0);
+    std::fprintf(f, "\n");
+    write_block(f, std::vector<double>(px.begin(), px.end()));
+    write_block(f, std::vector<double>(py.begin(), py.end()));
+    write_block(f, std::vector<double>(pz.begin(), pz.end()));
     for (int k = 0; k < F; k++) std::fprintf(f, "%d%c", fo[k + 1] - fo[k], (k % 16 == 15 || k == F - 1) ? '\n' : ' ');
     for (int k = 0; k < F; k++) {
         for (int q = fo[k]; q < fo[k + 1]; q++) std::fprintf(f, "%d ", fv[q] + 1);
@@ -463,6 +467,123 @@ extern "C" int plbfs_write_tecplot(const plbfs_mesh *m, const char *path, const 
     for (int k = 0; k < F; k++) std::fprintf(f, "%d%c", owner[k] + 1, (k % 16 == 15 || k == F - 1) ? '\n' : ' ');
     for (int k = 0; k < F; k++)
         std::fprintf(f, "%d%c", k < NI ? nb[k] + 1 : 0, (k % 16 == 15 || k == F - 1) ? '\n' : ' ');
+    std::fclose(f);
+    return PLBFS_OK;
+}
+
+extern "C" int plbfs_write_tecplot(const plbfs_mesh *m, const char *path, const double *rho, const double *u,
+                                   const double *v, const double *w, const double *res4, const double *dt_local,
+                                   double max_velocity, double visc, double gamma, double time) {
+    if (!m || !path || !rho || !u || !v || !w) return plbfs_fail(PLBFS_ERR_INPUT, "null argument");
+    if (!(max_velocity != 0) || !(gamma != 0)) return plbfs_fail(PLBFS_ERR_INPUT, "max_velocity and gamma");
+    int64_t info[6];
+    plbfs_mesh_info(m, info);
+    const int N = (int)info[0], F = (int)info[1], NI = (int)info[2], P = (int)info[3];
+    const size_t nfv = (size_t)plbfs_mesh_n_face_verts(m);
+    plbfs_mesh_view mv;
+    plbfs_mesh_view_get(m, &mv);
+    const int32_t *bp = plbfs_mesh_bface_patch(m);
+    // fneq: tau' = 3 nu / (dt_face gamma) of the last face a cell owns in face
+    // order, skipping the "empty" faces past the first boundary face
+    // (Solver.cpp:308,680: local_fneq[owner] is overwritten face by face); 0
+    // where a cell owns no such face (the reference leaves it unset)
+    std::vector<double> fneq(N, 0.0);
+    {
+        char buf[256];
+        std::vector<int> empty((size_t)info[4], 0);
+        for (int q = 0; q < (int)info[4]; q++) {
+            plbfs_mesh_patch_type(m, q, buf, sizeof buf);
+            empty[q] = std::strcmp(buf, "empty") == 0;
+        }
+        for (int fc = 0; fc < F; fc++) {
+            if (fc > NI && empty[bp[fc - NI]]) continue;
+            fneq[mv.owner[fc]] = visc * 3 / mv.face_dt[fc] / gamma;
+        }
+    }
+    FILE *f = std::fopen(path, "w");
+    if (!f) return plbfs_fail(PLBFS_ERR_INPUT, "cannot write %s", path);
+    std::fprintf(f,
+                 "TITLE = \"LBFS\"\nFILETYPE = SOLUTION\nVARIABLES = \"p\" \"u\" \"v\" \"w\" \"x\" \"y\" \"z\" "
+                 "\"vort\" \"stf\" \"res_rho\" \"res_u\" \"res_v\" \"res_w\" \"dt\" \"fneq\"\n");
+    polyhedral_zone_header(f, P, N, F, nfv, time);
+    std::fprintf(f, " VARLOCATION=([1-15]=CELLCENTERED)\n");
+    std::vector<double> col(N);
+    auto put = [&](auto get) {
+        for (int i = 0; i < N; i++) col[i] = get(i);
+        write_block(f, col);
+    };
+    // p is the density, velocities over max_velocity, x y z the centroids
+    put([&](int i) { return rho[i]; });
+    put([&](int i) { return u[i] / max_velocity; });
+    put([&](int i) { return v[i] / max_velocity; });
+    put([&](int i) { return w[i] / max_velocity; });
+    for (int a = 0; a < 3; a++) put([&](int i) { return mv.cell_centroid[3 * (size_t)i + a]; });
+    // vorticity and stream function stay at their initial zeros: the
+    // reference's calc_vorticity call is commented out (Solver.cpp:537,562)
+    put([&](int) { return 0.0; });
+    put([&](int) { return 0.0; });
+    for (int q = 0; q < 4; q++) put([&](int i) { return res4 ? res4[(size_t)4 * i + q] : 0.0; });
+    put([&](int i) { return dt_local ? dt_local[i] : 0.0; });
+    put([&](int i) { return fneq[i]; });
+    std::fclose(f);
+    return PLBFS_OK;
+}
+
+extern "C" int plbfs_write_tecplot_nodes(const char *path, const char *name, int32_t n, const double *x3,
+                                         const double *vel3, const double *force3, double X, double Y, double Z,
+                                         int32_t radial_nodes, int32_t depth_nodes, double time) {
+    if (!path || !name || n <= 0 || !x3 || !(X != 0 && Y != 0 && Z != 0))
+        return plbfs_fail(PLBFS_ERR_INPUT, "bad arguments");
+    FILE *f = std::fopen(path, "w");
+    if (!f) return plbfs_fail(PLBFS_ERR_INPUT, "cannot write %s", path);
+    std::fprintf(f, "TITLE = \"%s\"\nVARIABLES = \"nodx\" \"nody\" \"nodz\" \"vel_x\" \"vel_y\" \"vel_z\" "
+                 "\"force_x\" \"force_y\" \"force_z\"\n", name);
+    const bool poly = radial_nodes >= 2 && depth_nodes >= 2 && (int64_t)radial_nodes * depth_nodes == n;
+    const int R = radial_nodes, D = depth_nodes;
+    if (poly) {
+        // the cylinder surface as quads between consecutive rings, edges as
+        // faces (tecplot_output.cpp:784-1062)
+        std::fprintf(f,
+                     "ZONE T=\"%d\" ZONETYPE=FEPOLYGON NODES=%d ELEMENTS=%d FACES=%d NUMCONNECTEDBOUNDARYFACES=0 "
+                     "TOTALNUMBOUNDARYCONNECTIONS=0 DATAPACKING=BLOCK SOLUTIONTIME=%.17g\n",
+                     R * (D - 1), n, R * (D - 1), R * D + R * (D - 1), time);
+    } else {
+        std::fprintf(f, "ZONE T=\"%s\" I=%d DATAPACKING=BLOCK SOLUTIONTIME=%.17g\n", name, n, time);
+    }
+    // positions over the domain lengths, as the reference writes them
+    const double L3[3] = {X, Y, Z};
+    std::vector<double> col(n);
+    for (int a = 0; a < 3; a++) {
+        for (int k = 0; k < n; k++) col[k] = x3[3 * (size_t)k + a] / L3[a];
+        write_block(f, col);
+    }
+    for (const double *src : {vel3, force3})
+        for (int a = 0; a < 3; a++) {
+            for (int k = 0; k < n; k++) col[k] = src ? src[3 * (size_t)k + a] : 0.0;
+            write_block(f, col);
+        }
+    if (poly) {
+        // face nodes (1-based): ring edges, closing edge of each ring, then
+        // the axial edges between rings
+        for (int j = 0; j < D; j++) {
+            for (int k = 1; k < R; k++) std::fprintf(f, "%d %d\n", R * j + k, R * j + k + 1);
+            std::fprintf(f, "%d %d\n", R * (j + 1), 1 + R * j);
+        }
+        for (int j = 0; j < D - 1; j++)
+            for (int k = 0; k < R; k++) std::fprintf(f, "%d %d\n", R * j + k + 1, R * (j + 1) + k + 1);
+        // left and right elements of the faces in the same order (0: none)
+        std::vector<int> left, right;
+        for (int ii = 0; ii < R; ii++) left.push_back(0), right.push_back(ii + 1);
+        for (int j = 1; j < D - 1; j++)
+            for (int ii = 0; ii < R; ii++) left.push_back(R * (j - 1) + ii + 1), right.push_back(R * j + ii + 1);
+        for (int ii = 0; ii < R; ii++) left.push_back(R * (D - 2) + ii + 1), right.push_back(0);
+        for (int j = 0; j < D - 1; j++) {
+            left.push_back(R * j + 1), right.push_back(R * (j + 1));
+            for (int ii = 1; ii < R; ii++) left.push_back(R * j + ii + 1), right.push_back(R * j + ii);
+        }
+        for (int e : left) std::fprintf(f, "%d\n", e);
+        for (int e : right) std::fprintf(f, "%d\n", e);
+    }
     std::fclose(f);
     return PLBFS_OK;
 }
@@ -504,6 +625,7 @@ static int program_run(const char *xml_path, plbfs_result *result) {
     // the initial state over the ghosts (soln's entries past n_cells), which
     // some boundary types keep (src/Solution.cpp:759-833)
     if ((rc = plbfs_set_ghost_state(ctx, rho.data() + N, u.data() + N, v.data() + N, w.data() + N))) return rc;
+    std::vector<int32_t> obj_nodes;  // node count of each object, in node order
     if (!c->objects.empty()) {
         std::vector<double> all;
         double stiff = 0, area = 0;
@@ -515,6 +637,7 @@ static int program_run(const char *xml_path, plbfs_result *result) {
             std::vector<double> x((size_t)3 * n);
             if ((rc = plbfs_case_object_nodes(c, k, x.data(), n, &n, &s, &a, &ker))) return rc;
             all.insert(all.end(), x.begin(), x.end());
+            obj_nodes.push_back(n);
             stiff = s;  // the reference applies object 0's parameters to all nodes (Solver_gpu.cu:708-709)
             if (k == 0) {
                 area = a;
@@ -557,10 +680,42 @@ static int program_run(const char *xml_path, plbfs_result *result) {
     std::vector<double> sr(N), su(N), sv(N), sw(N);
     if ((rc = plbfs_get_state(ctx, sr.data(), su.data(), sv.data(), sw.data()))) return rc;
     if ((rc = plbfs_write_fields(outdir.c_str(), N, sr.data(), su.data(), sv.data(), sw.data()))) return rc;
+    // Tecplot: the grid once, the solution with the residual of the final
+    // state and its local time steps, and each immersed object's nodes
+    // (tecplot_output.cpp:273-466,784-1062; Solver_gpu.cu:911-912)
     mkdirs(outdir + "/plt");
-    const std::string plt = outdir + "/plt/" + std::to_string(r.total_steps) + ".dat";
-    if ((rc = plbfs_write_tecplot(m, plt.c_str(), sr.data(), su.data(), sv.data(), sw.data(), r.total_steps)))
-        return rc;
+    if ((rc = plbfs_write_tecplot_grid(m, (outdir + "/plt/grid.dat").c_str()))) return rc;
+    {
+        std::vector<double> res((size_t)4 * N), dtl(N);
+        if ((rc = plbfs_eval_residual(ctx, nullptr, res.data())) || (rc = plbfs_get_dt(ctx, dtl.data()))) return rc;
+        const std::string plt = outdir + "/plt/" + std::to_string(r.total_steps) + ".dat";
+        if ((rc = plbfs_write_tecplot(m, plt.c_str(), sr.data(), su.data(), sv.data(), sw.data(), res.data(),
+                                      dtl.data(), p.max_velocity, p.visc, p.gamma, r.total_steps)))
+            return rc;
+    }
+    if (!obj_nodes.empty()) {
+        int32_t ntot = 0;
+        for (int32_t n : obj_nodes) ntot += n;
+        std::vector<double> x((size_t)3 * ntot), vel((size_t)3 * ntot), force((size_t)3 * ntot);
+        if ((rc = plbfs_get_ibm(ctx, x.data(), vel.data(), force.data()))) return rc;
+        mkdirs(outdir + "/plt/object");
+        const double X = c->n("geometry/x", 1.0) * c->n("global_variables/scale", 1.0);
+        const double Y = c->n("geometry/y", 1.0) * c->n("global_variables/scale", 1.0);
+        const double Z = c->n("geometry/z", 1.0);
+        size_t k0 = 0;
+        for (size_t k = 0; k < obj_nodes.size(); k++) {
+            const ObjSpec &o = c->objects[k];
+            const int32_t n = obj_nodes[k];
+            // rings of a rigid cylinder (lagrangian_object.cpp:84-85); other objects are points
+            const int32_t rings = o.type == "rigid_cylinder" ? (int32_t)std::sqrt((double)o.num_nodes) : 0;
+            const std::string path = outdir + "/plt/object/" + o.name + std::to_string(r.total_steps) + ".dat";
+            if ((rc = plbfs_write_tecplot_nodes(path.c_str(), o.name.c_str(), n, x.data() + 3 * k0,
+                                                vel.data() + 3 * k0, force.data() + 3 * k0, X, Y, Z, rings, rings,
+                                                r.total_steps)))
+                return rc;
+            k0 += (size_t)n;
+        }
+    }
     // forces: every "wall" patch with the reference's coefficients
     // (src/post_processing.cpp:170-171), and the immersed objects
     if (FILE *f = std::fopen((outdir + "/drag.txt").c_str(), "w")) {

@@ -30,7 +30,8 @@ EXPORTS = [
     "plbfs_mesh_view_get", "plbfs_mesh_bface_patch", "plbfs_mesh_patch_type", "plbfs_mesh_raw",
     "plbfs_mesh_n_face_verts", "plbfs_case_load", "plbfs_case_free", "plbfs_case_params",
     "plbfs_case_string", "plbfs_case_number", "plbfs_case_patches", "plbfs_case_initial_state",
-    "plbfs_case_n_objects", "plbfs_case_object_nodes", "plbfs_write_tecplot", "plbfs_write_fields",
+    "plbfs_case_n_objects", "plbfs_case_object_nodes", "plbfs_write_tecplot", "plbfs_write_tecplot_grid",
+    "plbfs_write_tecplot_nodes", "plbfs_write_fields",
     "plbfs_read_fields", "plbfs_program_run",
     "plbfs_batch_create", "plbfs_batch_destroy", "plbfs_batch_set_state", "plbfs_batch_get_state",
     "plbfs_batch_set_ibm", "plbfs_batch_run", "plbfs_batch_set_tiling", "plbfs_batch_autotune",

@@ -75,10 +75,30 @@ int plbfs_case_object_nodes(const plbfs_case *c, int32_t k, double *x_ref3, int3
                             double *stiffness, double *node_area, int32_t *kernel);
 
 /* ---- Output ------------------------------------------------------------- */
-/* Tecplot ASCII (POINT/CELLCENTERED FEPOLYHEDRON zone) of the cell-centred
- * solution, and Lagrangian node zone; full-precision text fields rho/u/v/w.txt. */
+/* Tecplot ASCII in place of the reference's TecIO .plt files.  Grid file
+ * (FILETYPE=GRID): node coordinates and polyhedral faces, left element =
+ * owner + 1, right = neighbour + 1 or 0 (src/unstructured_mesh.cpp:2073-2200). */
+int plbfs_write_tecplot_grid(const plbfs_mesh *m, const char *path);
+/* Solution file (FILETYPE=SOLUTION) with the reference's cell-centred
+ * variables p u v w x y z vort stf res_rho res_u res_v res_w dt fneq
+ * (src/tecplot_output.cpp:273-466): p = rho, velocities over max_velocity,
+ * centroids, zero vorticity and stream function (their computation is
+ * commented out in the reference), the residual res4 [4*N] (rho, u, v, w per
+ * cell, as plbfs_eval_residual gives it; NULL: zeros), local time steps
+ * dt_local [N] (NULL: zeros) and tau' = 3 visc / (dt_face gamma) of the last
+ * face each cell owns (src/Solver.cpp:680). */
 int plbfs_write_tecplot(const plbfs_mesh *m, const char *path, const double *rho, const double *u, const double *v,
-                        const double *w, double time);
+                        const double *w, const double *res4, const double *dt_local, double max_velocity,
+                        double visc, double gamma, double time);
+/* One Lagrangian object (src/tecplot_output.cpp:784-1062): nodx nody nodz
+ * (positions over the domain lengths X, Y, Z) vel_x.. force_z per node; a
+ * rigid cylinder of radial_nodes x depth_nodes nodes as a FEPOLYGON zone with
+ * the reference's edge and element tables, any other node set (0, 0) as
+ * points.  vel3, force3 may be NULL (zeros). */
+int plbfs_write_tecplot_nodes(const char *path, const char *name, int32_t n, const double *x3, const double *vel3,
+                              const double *force3, double X, double Y, double Z, int32_t radial_nodes,
+                              int32_t depth_nodes, double time);
+/* Full-precision text fields rho/u/v/w.txt, the reference's "i ,value" lines. */
 int plbfs_write_fields(const char *dir, int32_t n, const double *rho, const double *u, const double *v,
                        const double *w);
 int plbfs_read_fields(const char *dir, int32_t n, double *rho, double *u, double *v, double *w);

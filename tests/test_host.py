@@ -271,17 +271,100 @@ def test_fields_roundtrip_full_precision(tmp_path):
     assert open(tmp_path / "rho.txt").readline().startswith("0 ,")
 
 
-def test_tecplot_ascii_zone(tmp_path):
-    pm = cavity_mesh(3, tets=True)
-    hm = HostMesh.from_polymesh(pm, 0.001)
+def _tecplot_numbers(text, skip_lines):
+    """Whitespace-separated numbers after the first skip_lines lines."""
+    return np.array([float(t) for t in " ".join(text.splitlines()[skip_lines:]).split()])
+
+
+def test_tecplot_grid_and_solution_files(tmp_path):
+    """Grid file: node coordinates and polyhedral faces.  Solution file: the
+    reference's 15 cell-centred variables (tecplot_output.cpp:347-348) in
+    order, p = rho, velocities over max_velocity, centroids, zero vort/stf,
+    the given residual and dt, and fneq = 3 nu / (dt_face gamma) of the last
+    face a cell owns (Solver.cpp:680), "empty" faces past face NI skipped."""
+    nu, gam, U = 0.004, 0.7, 0.05
+    pm = meshgen.box_mesh(3, 3, 3, lengths=(3.0, 3.0, 1.5), permute_seed=2,
+                          patch_types={"x0": "wall", "x1": "wall", "y0": "wall", "y1": "wall", "z0": "empty",
+                                       "z1": "empty"})
+    hm = HostMesh.from_polymesh(pm, nu, gam)
+    N, F, NI = pm.n_cells, pm.n_faces, pm.n_internal
     L = nat.lib()
-    L.plbfs_write_tecplot.argtypes = [C.c_void_p, C.c_char_p] + [C.POINTER(C.c_double)] * 4 + [C.c_double]
-    f = [np.linspace(0, 1, pm.n_cells) for _ in range(4)]
-    path = tmp_path / "sol.dat"
-    nat.check(L.plbfs_write_tecplot(hm.h, str(path).encode(), *(nat.dptr(x) for x in f), 3.0))
-    text = path.read_text()
-    assert "ZONETYPE=FEPOLYHEDRON" in text and "ELEMENTS=%d" % pm.n_cells in text
-    assert "FACES=%d" % pm.n_faces in text
+    L.plbfs_write_tecplot_grid.argtypes = [C.c_void_p, C.c_char_p]
+    L.plbfs_write_tecplot.argtypes = ([C.c_void_p, C.c_char_p] + [C.POINTER(C.c_double)] * 6 + [C.c_double] * 4)
+    grid = tmp_path / "grid.dat"
+    nat.check(L.plbfs_write_tecplot_grid(hm.h, str(grid).encode()))
+    gt = grid.read_text()
+    assert "FILETYPE = GRID" in gt and "ELEMENTS=%d FACES=%d" % (N, F) in gt
+    P = pm.points.shape[0]
+    nums = _tecplot_numbers(gt, 4)
+    assert np.array_equal(nums[:3 * P], np.asarray(pm.points, np.float32).T.reshape(-1).astype(np.float64))
+    rng = np.random.default_rng(1)
+    fields = [1 + 0.01 * rng.standard_normal(N)] + [U * rng.standard_normal(N) for _ in range(3)]
+    res = rng.standard_normal(4 * N)
+    dtl = rng.uniform(0.1, 0.5, N)
+    sol = tmp_path / "7.dat"
+    nat.check(L.plbfs_write_tecplot(hm.h, str(sol).encode(), *(nat.dptr(x) for x in fields), nat.dptr(res),
+                                    nat.dptr(dtl), U, nu, gam, 7.0))
+    st = sol.read_text()
+    assert st.splitlines()[2] == ('VARIABLES = "p" "u" "v" "w" "x" "y" "z" "vort" "stf" "res_rho" "res_u" '
+                                  '"res_v" "res_w" "dt" "fneq"')
+    assert "FILETYPE = SOLUTION" in st and "VARLOCATION=([1-15]=CELLCENTERED)" in st and "SOLUTIONTIME=7" in st
+    cols = _tecplot_numbers(st, 4).reshape(15, N)
+    geo = hm.geometry()
+    assert np.array_equal(cols[0], fields[0])
+    for q in (1, 2, 3):
+        assert np.array_equal(cols[q], fields[q] / U)
+    assert np.array_equal(cols[4], geo["cx"][:N]) and np.array_equal(cols[6], geo["cz"][:N])
+    assert not cols[7].any() and not cols[8].any()
+    for q in range(4):
+        assert np.array_equal(cols[9 + q], res.reshape(N, 4)[:, q])
+    assert np.array_equal(cols[13], dtl)
+    # faces of the 1 x 1 x 0.5 cells: x and y faces A = 0.5, z faces A = 1;
+    # dt_face = 0.5 sqrt(A); the z sides are empty, so their faces past NI
+    # do not set fneq
+    empty = np.isin(pm.bface_patch, [pm.patch_names.index("empty")])
+    want = np.zeros(N)
+    for f in range(F):
+        if f > NI and empty[f - NI]:
+            continue
+        want[pm.owner[f]] = nu * 3 / geo["dt"][f] / gam
+    assert np.allclose(cols[14], want, rtol=1e-15, atol=0)
+    # both face sizes occur, and cells whose owned faces are all empty keep 0
+    assert set(np.round(want, 12)) == {round(nu * 3 / (0.5 * np.sqrt(0.5)) / gam, 12),
+                                       round(nu * 3 / 0.5 / gam, 12), 0.0}
+
+
+def test_tecplot_cylinder_zone(tmp_path):
+    """A 3 x 2 rigid cylinder as the reference's FEPOLYGON zone
+    (tecplot_output.cpp:966-1057): ring edges, axial edges, and the elements
+    on their two sides; positions over the domain lengths."""
+    L = nat.lib()
+    L.plbfs_write_tecplot_nodes.argtypes = ([C.c_char_p, C.c_char_p, C.c_int32] + [C.POINTER(C.c_double)] * 3
+                                            + [C.c_double] * 3 + [C.c_int32, C.c_int32, C.c_double])
+    x = np.arange(18.0) + 1.0
+    vel = -np.arange(18.0)
+    path = tmp_path / "cyl.dat"
+    nat.check(L.plbfs_write_tecplot_nodes(str(path).encode(), b"cyl", 6, nat.dptr(x), nat.dptr(vel), None,
+                                          2.0, 4.0, 8.0, 3, 2, 1.0))
+    lines = path.read_text().splitlines()
+    assert "ZONETYPE=FEPOLYGON NODES=6 ELEMENTS=3 FACES=9" in lines[2]
+    nums = _tecplot_numbers(chr(10).join(lines), 3)
+    data = nums[:54].reshape(9, 6)
+    X = x.reshape(6, 3)
+    assert np.array_equal(data[0], X[:, 0] / 2.0) and np.array_equal(data[2], X[:, 2] / 8.0)
+    assert np.array_equal(data[3], vel.reshape(6, 3)[:, 0]) and not data[6:].any()
+    rest = nums[54:].astype(int)
+    edges = rest[:18].reshape(9, 2).tolist()
+    assert edges == [[1, 2], [2, 3], [3, 1], [4, 5], [5, 6], [6, 4], [1, 4], [2, 5], [3, 6]]
+    left, right = rest[18:27].tolist(), rest[27:36].tolist()
+    assert left == [0, 0, 0, 1, 2, 3, 1, 2, 3] and right == [1, 2, 3, 0, 0, 0, 3, 1, 2]
+    # element e is the quad of ring nodes e, e+1 (mod 3) on both rings: every
+    # edge lies on the elements it names
+    quad = {e: {e, e % 3 + 1, e + 3, e % 3 + 4} for e in (1, 2, 3)}
+    for (a, b), l_, r_ in zip(edges, left, right):
+        for e in (l_, r_):
+            if e:
+                assert {a, b} <= quad[e], (a, b, e)
 
 
 def test_create_without_device_is_a_clean_error():
