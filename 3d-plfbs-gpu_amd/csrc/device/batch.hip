@@ -444,7 +444,9 @@ int ibm_upload(plbfs_batch *b) {
     int32_t *nc = nullptr, *npo = nullptr, *npc = nullptr, *cpc = nullptr, *cpk = nullptr, *cpo = nullptr,
             *cpn = nullptr, *cpp = nullptr;
     double *kd = nullptr, *ccd = nullptr, *pw = nullptr;
+    double *xcd = nullptr;  // the lists were built around the positions given to set_ibm
     if ((rc = bupload(M, &a.xref, xr)) || (rc = bupload(M, &a.x, xx)) || (rc = bupload(M, &a.x0, xx)) ||
+        (rc = bupload(M, &xcd, xx)) ||
         (rc = bupload(M, &a.vel, z)) || (rc = bupload(M, &a.force, z)) || (rc = bupload(M, &a.norm, zn)) ||
         (rc = bupload(M, &kd, kda)) || (rc = bupload(M, &nc, ncase)) || (rc = bupload(M, &npo, np_off)) ||
         (rc = bupload(M, &npc, np_cell)) || (rc = bupload(M, &cpc, cp_cell)) || (rc = bupload(M, &cpk, cp_case)) ||
@@ -452,6 +454,8 @@ int ibm_upload(plbfs_batch *b) {
         (rc = bupload(M, &ccd, b->base->L.centroid)) ||
         (rc = balloc(M, (void **)&pw, np_cell.size() * sizeof(double))))
         return rc;
+    a.xc = xcd;
+    a.rebuild = 0;  // a batch case stops at the margin (PLBFS_ERR_IBM)
     a.node_kdA = kd;
     a.node_case = nc;
     a.np_off = npo;

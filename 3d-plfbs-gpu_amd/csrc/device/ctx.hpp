@@ -36,6 +36,9 @@ struct IbmState {
     int kernel = 0;
     double h = 1, dt = 0, stiffness = 0, node_area = 0, margin = 0;
     double *x = nullptr, *x0 = nullptr, *xref = nullptr, *vel = nullptr, *force = nullptr, *norm = nullptr;
+    double *xc = nullptr;            // [3][n] positions the candidate lists were built around
+    std::vector<double> xref_host;   // [3n] interleaved anchors, kept for list rebuilds
+    int rebuilds = 0;                // list rebuilds since plbfs_set_ibm
     int32_t *np_off = nullptr, *np_cell = nullptr;           // node -> candidate cells
     int32_t *cp_cell = nullptr, *cp_off = nullptr, *cp_node = nullptr;  // affected cell -> nodes
     int32_t *cp_pair = nullptr;  // node-major pair index of each cell-major entry
@@ -147,11 +150,15 @@ int create_ctx(const plbfs_mesh_view *mesh, const plbfs_params *params, const pl
                bool split_border, plbfs_ctx **out);
 
 // Launches the per-step immersed-boundary kernels (ibm.hip) against the control
-// block ctrl; no-op when off.  timed: bracket them with timing events.
-int ibm_launch(plbfs_ctx *c, plbfs::Ctrl *ctrl, bool timed);
+// block ctrl; no-op when off.  timed: bracket them with timing events;
+// rebuild: flag drifted nodes for a list rebuild (ctrl->rebuild).
+int ibm_launch(plbfs_ctx *c, plbfs::Ctrl *ctrl, bool timed, bool rebuild);
+// Rebuilds the candidate lists around the nodes' current positions, as
+// plbfs_set_ibm with those positions would (ibm.hip).
+int ibm_rebuild(plbfs_ctx *c);
 
 // Node-major candidate cells of every node (centroid within the kernel support
-// plus a drift margin of the anchor) and the same pairs regrouped by cell.
+// plus a drift margin of the node's position) and the same pairs regrouped by cell.
 struct IbmPairs {
     double margin = 0;
     std::vector<int32_t> np_off, np_cell;          // node -> candidate cells (sorted)

@@ -29,9 +29,11 @@ using namespace plbfs;
 
 namespace {
 
-IbmArgs make_args(plbfs_ctx *c) {
+IbmArgs make_args(plbfs_ctx *c, bool rebuild = false) {
     const IbmState &s = c->ibm;
     IbmArgs a;
+    a.xc = s.xc;
+    a.rebuild = rebuild ? 1 : 0;
     a.n = s.n;
     a.N = c->L.N;
     a.B = 1;
@@ -66,9 +68,9 @@ IbmArgs make_args(plbfs_ctx *c) {
 
 }  // namespace
 
-int ibm_launch(plbfs_ctx *c, Ctrl *ctrl, bool timed) {
+int ibm_launch(plbfs_ctx *c, Ctrl *ctrl, bool timed, bool rebuild) {
     if (!c->ibm.on) return PLBFS_OK;
-    const IbmArgs a = make_args(c);
+    const IbmArgs a = make_args(c, rebuild);
     TimedLaunch t(c, PLBFS_KSTAT_IBM, timed);
     const dim3 grid((a.n + kNodesPerBlock - 1) / kNodesPerBlock);
     if (c->f32)
@@ -90,23 +92,21 @@ int build_ibm_pairs(const plbfs::Layout &L, const plbfs_ibm_nodes *nodes, IbmPai
     const int N = L.N;
     const double h = nodes->h;
     const double support = (nodes->kernel == PLBFS_DELTA_HAT2 ? 1.0 : 2.0) * h;
-    // drift margin: the penalty nodes creep slowly downstream (by ~0.5 h over
-    // 2e5 steps on the 4.1M-tet sphere case, which stopped at a 0.5 h margin)
+    // drift margin around the position a node's list is built at: a node that
+    // moves half of it gets a new list (plbfs_run); one that moves all of it
+    // may miss cells and stops the run
     const double margin = 1.0 * h;
     const double reach = support + margin;
     P.margin = margin;
     const double *cc = L.centroid.data();
-    for (int k = 0; k < n; k++)
-        for (int q = 0; q < 3; q++)
-            if (std::fabs(nodes->x[3 * k + q] - nodes->x_ref[3 * k + q]) > margin)
-                return plbfs_fail(PLBFS_ERR_IBM, "node %d starts farther than %g from its anchor", k, margin);
+    const double *xs = nodes->x;  // lists around the nodes' current positions
     // bin the cells near the node cloud
     double lo[3], hi[3];
     for (int q = 0; q < 3; q++) {
-        lo[q] = hi[q] = nodes->x_ref[q];
+        lo[q] = hi[q] = xs[q];
         for (int k = 0; k < n; k++) {
-            lo[q] = std::min(lo[q], nodes->x_ref[3 * k + q]);
-            hi[q] = std::max(hi[q], nodes->x_ref[3 * k + q]);
+            lo[q] = std::min(lo[q], xs[3 * k + q]);
+            hi[q] = std::max(hi[q], xs[3 * k + q]);
         }
         lo[q] -= reach;
         hi[q] += reach;
@@ -125,7 +125,7 @@ int build_ibm_pairs(const plbfs::Layout &L, const plbfs_ibm_nodes *nodes, IbmPai
     np_cell.clear();
     std::vector<std::array<int32_t, 3>> cell_node;  // (cell, node, node-major pair)
     for (int k = 0; k < n; k++) {
-        const double *X = nodes->x_ref + 3 * k;
+        const double *X = xs + 3 * k;
         const int64_t bx = (int64_t)std::floor((X[0] - lo[0]) / reach), by = (int64_t)std::floor((X[1] - lo[1]) / reach),
                       bz = (int64_t)std::floor((X[2] - lo[2]) / reach);
         std::vector<int32_t> cand;
@@ -166,8 +166,7 @@ int build_ibm_pairs(const plbfs::Layout &L, const plbfs_ibm_nodes *nodes, IbmPai
     return PLBFS_OK;
 }
 
-extern "C" int plbfs_set_ibm(plbfs_ctx *c, const plbfs_ibm_nodes *nodes) {
-    if (!c || !nodes) return plbfs_fail(PLBFS_ERR_INPUT, "null argument");
+static int set_ibm(plbfs_ctx *c, const plbfs_ibm_nodes *nodes, int rebuilds) {
     IbmPairs P;
     int rc = build_ibm_pairs(c->L, nodes, P);
     if (rc) return rc;
@@ -203,6 +202,8 @@ extern "C" int plbfs_set_ibm(plbfs_ctx *c, const plbfs_ibm_nodes *nodes) {
     s.margin = margin;
     s.n_affected = (int)cp_cell.size();
     s.n_pairs = (int64_t)np_cell.size();
+    s.xref_host.assign(nodes->x_ref, nodes->x_ref + 3 * (size_t)n);
+    s.rebuilds = rebuilds;
     std::vector<double> xr((size_t)3 * n), xx((size_t)3 * n), z((size_t)3 * n, 0.0), zn(n, 0.0);
     for (int k = 0; k < n; k++)
         for (int q = 0; q < 3; q++) {
@@ -210,6 +211,7 @@ extern "C" int plbfs_set_ibm(plbfs_ctx *c, const plbfs_ibm_nodes *nodes) {
             xx[(size_t)q * n + k] = nodes->x[3 * k + q];
         }
     if ((rc = upload(M, &s.xref, xr)) || (rc = upload(M, &s.x, xx)) || (rc = upload(M, &s.x0, xx)) ||
+        (rc = upload(M, &s.xc, xx)) ||
         (rc = upload(M, &s.vel, z)) || (rc = upload(M, &s.force, z)) || (rc = upload(M, &s.norm, zn)) ||
         (rc = upload(M, &s.np_off, np_off)) || (rc = upload(M, &s.np_cell, np_cell)) ||
         (rc = upload(M, &s.cp_cell, cp_cell)) || (rc = upload(M, &s.cp_off, cp_off)) ||
@@ -240,6 +242,23 @@ extern "C" int plbfs_set_ibm(plbfs_ctx *c, const plbfs_ibm_nodes *nodes) {
     }
     c->graph_ok = false;
     return PLBFS_OK;
+}
+
+extern "C" int plbfs_set_ibm(plbfs_ctx *c, const plbfs_ibm_nodes *nodes) {
+    if (!c || !nodes) return plbfs_fail(PLBFS_ERR_INPUT, "null argument");
+    return set_ibm(c, nodes, 0);
+}
+
+int ibm_rebuild(plbfs_ctx *c) {
+    if (!c->ibm.on) return PLBFS_OK;
+    const IbmState &s = c->ibm;
+    const int n = s.n;
+    std::vector<double> x((size_t)3 * n);
+    int rc = plbfs_get_ibm(c, x.data(), nullptr, nullptr);
+    if (rc) return rc;
+    const std::vector<double> xr = s.xref_host;
+    const plbfs_ibm_nodes nodes{n, xr.data(), x.data(), s.stiffness, s.node_area, s.dt, s.h, s.kernel};
+    return set_ibm(c, &nodes, s.rebuilds + 1);
 }
 
 extern "C" int plbfs_get_ibm(plbfs_ctx *c, double *x, double *vel, double *force) {

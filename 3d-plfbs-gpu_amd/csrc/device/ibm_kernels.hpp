@@ -50,6 +50,8 @@ struct IbmArgs {
     int B;  // cases per cell record (1 for a single case)
     double ih, dt, k_dA, margin;
     double *x, *x0, *xref, *vel, *force, *norm;
+    const double *xc;        // [3][n] positions the candidate lists were built around
+    int rebuild;             // 1: ask the host for new lists at half the margin (single case)
     const double *node_kdA;  // per-node k dA, or null for the scalar k_dA
     const int32_t *node_case;  // per-node case, or null (case 0)
     const int32_t *np_off, *np_cell;
@@ -104,7 +106,7 @@ __global__ __launch_bounds__(kIbmThreads) void k_ibm_nodes(IbmArgs a, Ctrl *__re
         const double x0 = a.x0[(size_t)q * n + k], xr = a.xref[(size_t)q * n + k];
         Xn[q] = x0 + U[q] * a.dt;
         F[q] = (-kdA * (Xn[q] - xr));
-        drift = fmax(drift, fabs(Xn[q] - xr));
+        drift = fmax(drift, fabs(Xn[q] - a.xc[(size_t)q * n + k]));
     }
     double S2 = 0;
     for (int p = p0 + lane; p < p1; p += 64) {
@@ -123,6 +125,9 @@ __global__ __launch_bounds__(kIbmThreads) void k_ibm_nodes(IbmArgs a, Ctrl *__re
             a.force[(size_t)q * n + k] = F[q];
         }
         a.norm[k] = S2;
+        // past half the margin the lists are rebuilt around the new positions
+        // after this step; past the margin they may miss cells: an error
+        if (a.rebuild && drift > 0.5 * a.margin) ctrl->rebuild = 1;
         if (!supported || !(S2 > 0.0) || drift > a.margin) {
             if (a.case_status) {
                 // batch: only this case stops

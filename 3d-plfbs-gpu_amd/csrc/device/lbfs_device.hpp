@@ -34,6 +34,7 @@ struct Ctrl {
     int32_t status;  // 0 ok, PLBFS_ERR_NAN, PLBFS_ERR_RHO, PLBFS_ERR_IBM
     int32_t step;    // steps completed in this run
     int32_t converged;
+    int32_t rebuild;  // nonzero: an immersed node drifted half its list margin
     double res[4];
 };
 

@@ -258,7 +258,8 @@ __global__ __launch_bounds__(kBlock) void k_init_dt(int N, DevParams prm, const 
 }
 
 __global__ __launch_bounds__(kFinalizeThreads) void k_finalize(int nblk, double n_cells, double ref_rho,
-                                                               double ref_mom, double tol, Ctrl *__restrict__ ctrl,
+                                                               double ref_mom, double tol, int stop_on_rebuild,
+                                                               Ctrl *__restrict__ ctrl,
                                                                const double *__restrict__ partials,
                                                                double *__restrict__ hist, int hist_cap) {
     if (ctrl->done) return;
@@ -300,6 +301,9 @@ __global__ __launch_bounds__(kFinalizeThreads) void k_finalize(int nblk, double 
                 ctrl->done = 1;
             }
         }
+        // immersed nodes asked for new candidate lists: the host builds them
+        // and continues with the next step
+        if (stop_on_rebuild && ctrl->rebuild) ctrl->done = 1;
     }
 }
 
@@ -822,9 +826,12 @@ static int launch_run_start(plbfs_ctx *c, const RunScratch &w) {
     return PLBFS_OK;
 }
 
-static int launch_step(plbfs_ctx *c, const RunScratch &w, bool timed) {
+// rebuild: immersed nodes that drift half their list margin end the step
+// replay so that the host rebuilds the lists (plbfs_run); without it they run
+// on to the margin (plbfs_run_noalloc, which cannot allocate new lists).
+static int launch_step(plbfs_ctx *c, const RunScratch &w, bool timed, bool rebuild) {
     const int nb = c->L.n_blocks;
-    int rc = ibm_launch(c, w.ctrl, timed);  // once per step, before the stages (Solver_gpu.cu:687-718)
+    int rc = ibm_launch(c, w.ctrl, timed, rebuild);  // once per step, before the stages (Solver_gpu.cu:687-718)
     if (rc) return rc;
     if (c->f32) launch_stages<float>(c, c->dm32, w, timed);
     else launch_stages<double>(c, c->dm, w, timed);
@@ -832,7 +839,8 @@ static int launch_step(plbfs_ctx *c, const RunScratch &w, bool timed) {
     {
         TimedLaunch t(c, PLBFS_KSTAT_REDUCE, timed);
         hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kFinalizeThreads), 0, c->stream, nb, (double)c->L.N,
-                           c->prm.ref_rho, c->prm.ref_mom, c->prm.tol, w.ctrl, w.partials, w.hist, w.hist_cap);
+                           c->prm.ref_rho, c->prm.ref_mom, c->prm.tol, rebuild ? 1 : 0, w.ctrl, w.partials,
+                           w.hist, w.hist_cap);
     }
     HIP_TRY(hipGetLastError());
     return PLBFS_OK;
@@ -856,7 +864,7 @@ static int build_graph(plbfs_ctx *c) {
     if (c->graph_ok || !c->use_graph || c->timing) return PLBFS_OK;
     hipGraph_t g;
     HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    int rc = launch_step(c, ctx_scratch(c), false);
+    int rc = launch_step(c, ctx_scratch(c), false, true);
     hipError_t e = hipStreamEndCapture(c->stream, &g);
     if (rc) return rc;
     if (e != hipSuccess) return plbfs_fail(PLBFS_ERR_HIP, "graph capture: %s", hipGetErrorString(e));
@@ -897,12 +905,23 @@ extern "C" int plbfs_run(plbfs_ctx *c, int32_t max_steps, plbfs_result *out) {
     for (int s = 0; s < max_steps; s++) {
         if (c->graph_ok && !c->timing) {
             HIP_TRY(hipGraphLaunch(c->graph_exec, c->stream));
-        } else if ((rc = launch_step(c, w, c->timing))) {
+        } else if ((rc = launch_step(c, w, c->timing, true))) {
             return rc;
         }
         if ((s + 1) % interval == 0 || s == max_steps - 1) {
             HIP_TRY(hipMemcpyAsync(&h, c->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, c->stream));
             HIP_TRY(hipStreamSynchronize(c->stream));
+            if (h.done && h.rebuild && h.status == PLBFS_OK && !h.converged) {
+                // new candidate lists around the nodes' positions after step
+                // h.step, then on from there: the launches after the stop were
+                // no-ops, so the loop resumes at the step count
+                if ((rc = ibm_rebuild(c)) || (rc = ensure_hist(c, max_steps)) || (rc = build_graph(c))) return rc;
+                h.done = 0;
+                h.rebuild = 0;
+                HIP_TRY(hipMemcpyAsync(c->ctrl, &h, sizeof(Ctrl), hipMemcpyHostToDevice, c->stream));
+                s = h.step - 1;
+                continue;
+            }
             if (h.done) break;
         }
     }
@@ -960,7 +979,7 @@ extern "C" int plbfs_run_noalloc(plbfs_ctx *c, int32_t max_steps, plbfs_result *
     const int interval = std::max(1, c->params.check_interval);
     Ctrl h{};
     for (int s = 0; s < max_steps; s++) {
-        if ((rc = launch_step(c, w, false))) return rc;
+        if ((rc = launch_step(c, w, false, false))) return rc;
         if ((s + 1) % interval == 0 || s == max_steps - 1) {
             HIP_TRY(hipMemcpyAsync(&h, w.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, c->stream));
             HIP_TRY(hipStreamSynchronize(c->stream));

@@ -64,12 +64,59 @@ def test_ibm_force_conservation(gpu_available, kernel):
     assert np.allclose((f * vol[:, None]).sum(0), F.sum(0), rtol=1e-12, atol=1e-16)
 
 
-def test_ibm_rejects_far_nodes(gpu_available):
+def test_ibm_rejects_nodes_without_cells(gpu_available):
+    # candidate lists are built around the nodes' positions, which may lie
+    # anywhere relative to their anchors; a node with no cell within its
+    # kernel support plus margin is an error
     pm, o, s, X = setup(False, 1)
     from plbfs import _native as nat
+    s.set_ibm(X, X + 2.0, stiffness=1.0, node_area=1.0, dt=0.5, h=1.0, kernel=1)
     with pytest.raises(nat.PlbfsError) as e:
-        s.set_ibm(X, X + 5.0, stiffness=1.0, node_area=1.0, dt=0.5, h=1.0, kernel=1)
+        s.set_ibm(X, X + 50.0, stiffness=1.0, node_area=1.0, dt=0.5, h=1.0, kernel=1)
     assert e.value.rc == -7
+
+
+def test_ibm_lists_follow_drifting_nodes(gpu_available):
+    """Nodes that drift half their list margin (0.5 h) get new candidate lists
+    built around their positions, so a sphere carried 3 h downstream runs
+    without PLBFS_ERR_IBM; the run equals, bit for bit, one that stops after
+    every step and calls plbfs_set_ibm afresh whenever a node has drifted that
+    far (the FP64 local time step restarts from the same state bits)."""
+    pm = channel_mesh(20, 8, 8, jitter=0.1)
+    case = pc.channel_case(re=40.0, length_scale=3.0)
+    case.check_interval = 25
+    X, dA = objects.fibonacci_sphere(60, (4.0, 4.0, 4.0), 1.5)
+    # a weak tether: the nodes are carried with the stream, and their small
+    # spring force still reaches the fields through the spreading
+    ibm = dict(stiffness=1e-4, node_area=dA, dt=case.time_marching_step, h=1.0, kernel=1)
+    runs = []
+    for stepwise in (False, True):
+        hm = HostMesh.from_polymesh(pm, case.visc, case.gamma, case.fneq_min)
+        s = GpuSolver(hm, case.params, case.patches_for(pm.patch_names))
+        cells = hm.geometry()
+        s.set_state(*case.initial_state(cells["cx"], cells["cy"], cells["cz"]))
+        s.set_ibm(X, X, **ibm)
+        rebuilds = 0
+        if not stepwise:
+            r = s.run(200)
+            assert r["rc"] == 0 and r["steps"] == 200, r
+        else:
+            centre = X.copy()
+            for t in range(200):
+                assert s.run(1)["rc"] == 0, t
+                x, _, _ = s.ibm_state()
+                if np.abs(x - centre).max() > 0.5:
+                    s.set_ibm(X, x, **ibm)
+                    centre = x.copy()
+                    rebuilds += 1
+        runs.append((s.state(), s.ibm_state(), rebuilds))
+    (sa, ia, _), (sb, ib, nb) = runs
+    assert nb >= 4, nb
+    assert (ia[0] - X)[:, 0].min() > 3.0  # every node carried more than 3 h downstream
+    for q in range(4):
+        assert np.array_equal(sa[q], sb[q]), q
+    for q in range(3):
+        assert np.array_equal(ia[q], ib[q]), q
 
 
 def test_sphere_drag_points_downstream(gpu_available):
