@@ -21,7 +21,11 @@ namespace plbfs {
 constexpr int kBlock = 256;
 // default cells per flux block (face slots of one block share its LDS)
 constexpr int kFluxBlockCells = 256;
-constexpr int kMaxLds = 64 * 1024;  // dynamic LDS per workgroup used by the flux kernels
+// Dynamic LDS per flux workgroup.  gfx950 gives a workgroup 160 KiB
+// (hipDeviceProp_t::sharedMemPerBlock), static and dynamic LDS together; the
+// flux kernels' static arrays take at most 160 B, and a launch whose total
+// exceeds 160 KiB is not refused by the runtime but faults, so 1 KiB stays free.
+constexpr int kMaxLds = 159 * 1024;
 }
 
 struct KStat {

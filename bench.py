@@ -98,7 +98,24 @@ def cpu_baseline(pm, case, st, seconds_hint):
     assert r["rc"] == 0, r["rc"]
     return dict(value=pm.n_cells / dt, unit="cell-updates/s", cores=1, kind="port",
                 sample="1 time step (4 RK stages) of the same %d-cell mesh by the single-threaded FP64 C oracle "
-                       "(restatement of src/Solver.cpp), %.1f s" % (pm.n_cells, dt))
+                       "(restatement of src/Solver.cpp), %.1f s" % (pm.n_cells, dt),
+                host=host_cpu())
+
+
+def host_cpu():
+    """CPU model of the host the baseline ran on, its logical CPUs and the
+    ones this process may use."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return dict(model=model, nproc=os.cpu_count(), usable=usable)
 
 
 def run_batch(args):
@@ -122,9 +139,15 @@ def run_batch(args):
         bt.set_ibm(b, **ws.ibm_args(d, c))
         nodes += d.n_nodes
     t = time.perf_counter()
-    bc, cp, sg, tune_ms = bt.autotune(3)
-    print("autotune: block_cells %d cases_per_block %d stage %d (%.3f ms/step, %.1f s)" % (
-        bc, cp, sg, tune_ms, time.perf_counter() - t), file=sys.stderr, flush=True)
+    if args.tiling:
+        bc, cp, sg = (int(x) for x in args.tiling.split(","))
+        bt.set_tiling(bc, cp, sg)
+        print("tiling: block_cells %d cases_per_block %d stage %d (given)" % (bc, cp, sg), file=sys.stderr,
+              flush=True)
+    else:
+        bc, cp, sg, tune_ms = bt.autotune(3)
+        print("autotune: block_cells %d cases_per_block %d stage %d (%.3f ms/step, %.1f s)" % (
+            bc, cp, sg, tune_ms, time.perf_counter() - t), file=sys.stderr, flush=True)
     if args.warmup > 0:
         bt.run(args.warmup)
     rc, res = bt.run(args.steps)
@@ -230,6 +253,8 @@ def main():
     ap.add_argument("--precision", choices=("fp64", "fp32"), default="fp64",
                     help="storage precision of fields, gradients and face data")
     ap.add_argument("--cases", type=int, default=128, help="cases of the batch128 workload")
+    ap.add_argument("--tiling", default="", help="batch128: cells per block, cases per workgroup, staging "
+                                                 "(e.g. 64,8,0) instead of the autotuner")
     ap.add_argument("--time-to-tol", action="store_true",
                     help="measure wall time to the residual tolerance (FP64 and FP32) instead of throughput")
     ap.add_argument("--tol", type=float, default=1e-6)
@@ -251,6 +276,8 @@ def main():
         print("bench.py: no HIP device visible", file=sys.stderr)
         sys.exit(1)
     if args.copy_calibration:
+        # a 1 GiB device copy; with HSA_ENABLE_SDMA=0 the runtime runs it as
+        # its blit kernel, whose dispatches the counters see
         import torch
         a = torch.empty(1 << 27, dtype=torch.float64, device="cuda")
         b = torch.empty_like(a)
