@@ -73,7 +73,7 @@ def algorithmic_bytes(info, P=8, Gb=8):
     return grad, flux
 
 
-def measured_traffic(workload, precision, block_cells):
+def measured_traffic(workload, precision, block_cells, suffix=""):
     """HBM bytes per launch of the flux kernel from the committed rocprofv3
     FETCH_SIZE / WRITE_SIZE summary (calibrated against a device copy), or
     None when no summary matches this workload, precision and block size."""
@@ -82,7 +82,7 @@ def measured_traffic(workload, precision, block_cells):
         return None
     with open(path) as f:
         table = json.load(f)
-    key = "%s/%s/b%d" % (workload, precision, block_cells)
+    key = "%s/%s/b%d%s" % (workload, precision, block_cells, suffix)
     return table.get(key)
 
 
@@ -172,9 +172,66 @@ def run_batch(args):
                    "staged": bool(sg)},
         "roofline": {"bound": "hbm", "achieved": step_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": step_gbs / HBM_PEAK_GBS, "traffic": None, "kernel": "whole step (graph replay)",
-                     "alg_bytes_per_case_cell_update": b_update},
+                     "alg_bytes_per_case_cell_update": b_update,
+                     # rocprofv3 FETCH/WRITE bytes of one kb_flux launch at this tiling, when measured
+                     "kb_flux_traffic": measured_traffic("batch128", args.precision, bc, "c%d" % cp)},
         "cpu_baseline": None,
     }
+    print(json.dumps(out))
+
+
+def run_batch_to_tolerance(args):
+    """The 128 batched cases (the seeded draws of run_batch, FP32 storage)
+    each run to max(u, v, w) residual args.tol on the shared 64 x 32 x 32
+    jittered hex channel, in chunks of args.chunk steps; a case freezes when it
+    converges.  Reports every case's draw, steps, status and the wall time of
+    the chunk in which it froze, and the whole batch's wall time."""
+    from plbfs import warmstart as ws
+    from plbfs.solver import GpuBatch
+    B = args.cases
+    pm = meshgen.box_mesh(64, 32, 32, lengths=(64.0, 32.0, 32.0), jitter=0.15, jitter_seed=12, permute_seed=13,
+                          patch_types=pc.CHANNEL_PATCHES)
+    hm = HostMesh.from_polymesh(pm, 0.01, 1.0, 0.0)
+    draws = ws.sample_cases(B, (64.0, 32.0, 32.0), seed=2024)
+    cases = [ws.case_for(d, precision=1, tol=args.tol, check=100) for d in draws]
+    bt = GpuBatch(hm, cases[0].params, cases[0].patches_for(pm.patch_names), [c.visc for c in cases])
+    bt.set_state(-1, *ws.uniform_state(cases[0], hm.geometry(), pm.n_cells))
+    for b, (d, c) in enumerate(zip(draws, cases)):
+        bt.set_ibm(b, **ws.ibm_args(d, c))
+    if args.tiling:
+        bt.set_tiling(*(int(x) for x in args.tiling.split(",")))
+    else:
+        bt.autotune(3)
+    info = bt.layout_info()
+    froze = [None] * B
+    t0 = time.perf_counter()
+    done_steps = 0
+    res = None
+    while done_steps < args.max_steps:
+        n = min(args.chunk, args.max_steps - done_steps)
+        rc, res = bt.run(n)
+        done_steps += n
+        wall = time.perf_counter() - t0
+        for b, r in enumerate(res):
+            if froze[b] is None and (r["converged"] or r["status"] != 0):
+                froze[b] = wall
+        live = sum(1 for f in froze if f is None)
+        print("  %d steps, %.1f s, %d of %d cases still running" % (done_steps, wall, live, B), file=sys.stderr,
+              flush=True)
+        if live == 0:
+            break
+    wall = time.perf_counter() - t0
+    rows = [dict(case=b, re=d.re, diameter=d.diameter, nodes=d.n_nodes, steps=r["steps"], converged=r["converged"],
+                 status=r["status"], residual=max(r["residual"][1:]), wall_s=froze[b])
+            for b, (d, r) in enumerate(zip(draws, res))]
+    conv = [r for r in rows if r["converged"]]
+    out = {"workload": "%d cases on a jittered 64x32x32 hex channel, FP32, each to residual %g" % (B, args.tol),
+           "tolerance": args.tol, "block_cells": info["block_cells"], "cases_per_block": info["cases_per_block"],
+           "wall_s": wall, "steps_run": done_steps, "converged": len(conv), "cases": rows}
+    if conv:
+        st = sorted(r["steps"] for r in conv)
+        out["steps_median"] = st[len(st) // 2]
+        out["steps_max"] = st[-1]
     print(json.dumps(out))
 
 
@@ -258,6 +315,7 @@ def main():
     ap.add_argument("--time-to-tol", action="store_true",
                     help="measure wall time to the residual tolerance (FP64 and FP32) instead of throughput")
     ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--chunk", type=int, default=1000, help="batch128 --time-to-tol: steps between progress reads")
     ap.add_argument("--max-steps", type=int, default=60000)
     ap.add_argument("--params", default="", help="trained warm-start parameters (.npy); default plbfs/data")
     ap.add_argument("--probe", type=int, default=200, help="steps of the warm-start fallback probe")
@@ -286,7 +344,7 @@ def main():
         print("copy calibration: 1073741824 B read + written", file=sys.stderr, flush=True)
 
     if args.workload == "batch128":
-        return run_batch(args)
+        return run_batch_to_tolerance(args) if args.time_to_tol else run_batch(args)
     if args.time_to_tol:
         return run_to_tolerance(args)
     t0 = time.perf_counter()
