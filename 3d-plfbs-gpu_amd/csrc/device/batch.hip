@@ -97,24 +97,27 @@ __global__ __launch_bounds__(kBlock) void kb_flux(DevMesh<S> m, DevParams prm, B
         nst = ncell;
         __syncthreads();
     }
-    const int nwork = (e1 - e0) * C;
-    for (int k = threadIdx.x; k < nwork; k += kBlock) {
-        const int el = k / C, c = k - el * C, b = b0 + c;
-        if (b >= B || a.done[b]) continue;
-        S F[4];
+    // kBlock is a multiple of C: thread t serves case lane c = t % C, items
+    // t / C, t / C + kBlock / C, ..., so its case's flags and viscosity are
+    // read once
+    const int c = threadIdx.x % C, b = b0 + c;
+    const bool live = b < B && !a.done[b];
+    if (live) {
+        const double v3g = a.v3g[b], dtcap = a.dtcap[b];
         const SideLoader<S, RecB, false> ld{st_in, grad, m.cgeo, RecB{B, b}, sRec, sCg, c0, nst, C, c};
-        item_flux<S>(m, prm, e0 + el, ld, a.v3g[b], a.dtcap[b], F);
+        for (int el = threadIdx.x / C; el < e1 - e0; el += kBlock / C) {
+            S F[4];
+            item_flux<S>(m, prm, e0 + el, ld, v3g, dtcap, F);
 #pragma unroll
-        for (int q = 0; q < 4; q++) sF[q * stride + el * C + c] = F[q];
+            for (int q = 0; q < 4; q++) sF[q * stride + el * C + c] = F[q];
+        }
     }
     __syncthreads();
     double sq[4] = {0, 0, 0, 0};
     int err = 0;
     const size_t NR = (size_t)N * B;
-    // kBlock is a multiple of C, so every thread serves the one case c = tid % C
-    for (int k = threadIdx.x; k < ncell * C; k += kBlock) {
-        const int il = k / C, c = k - il * C, b = b0 + c;
-        if (b >= B || a.done[b]) continue;
+    const double visc = live ? a.visc[b] : 0.0;
+    for (int il = threadIdx.x / C; live && il < ncell; il += kBlock / C) {
         const int i = c0 + il;
         double R[4];
         gather_residual<S>(m, i, sF, stride, C, c, R);
@@ -124,7 +127,7 @@ __global__ __launch_bounds__(kBlock) void kb_flux(DevMesh<S> m, DevParams prm, B
             R[2] += fbody[NR + r];
             R[3] += fbody[2 * NR + r];
         }
-        const int e = rk_update<S>(m, prm, a.visc[b], sa, i, r, R, st_out, t0, t1, dtl, sq);
+        const int e = rk_update<S>(m, prm, visc, sa, i, r, R, st_out, t0, t1, dtl, sq);
         if (e == 1) err = 1;
         else if (e == 2 && err == 0) err = 2;
     }

@@ -10,7 +10,9 @@
 // the reference's flux sums are exactly
 //   sum_k e_k f_k       = (1 - tau) rho_I u_I + tau S1      (= S1, since rho_I u_I = S1)
 //   sum_k e_k e_k^T f_k = (1 - tau) rho_I (I/3 + u_I u_I^T / gamma) + tau Pi
-// so only 10 running sums live in registers.
+// so only S0, S1 and Pi are needed.  They are accumulated as unweighted
+// pair sums per lattice weight class (13 values in registers) and weighted
+// once per face.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -226,28 +228,32 @@ __device__ __forceinline__ void ghost_grad(const DevPatch &P, const T n[3], T dm
 // Lattice evaluation (src/Solver.cpp:840-1374 restructured as moments)
 // ---------------------------------------------------------------------------
 
+// Unweighted pair sums h = feq / w.  Each axis pair (weight 1/9) feeds
+// exactly one component of each moment, so its sum S and difference D are kept
+// as they are; the four body diagonals (weight 1/72) all feed S0 and every
+// diagonal of Pi with the same sum, which is therefore accumulated once.  The
+// weights are applied once per face.
 template <typename T>
-struct Moments {
-    T s0, sx, sy, sz, pxx, pyy, pzz, pxy, pxz, pyz;
+struct PairSums {
+    T ax_s[3], ax_d[3];           // axis pairs x, y, z: S and D
+    T dg_s, dg_x, dg_y, dg_z;     // diagonals: sum S, sum e_a D
+    T dg_xy, dg_xz, dg_yz;        // diagonals: sum e_a e_b S
 };
 
-// Equilibrium of one lattice point e at state (r, u, v, w).  With DEV (FP32
-// storage) the density is a deviation r = rho - rho0 and the value returned is
-// g = feq - w rho0 = w (r (1 + X) + rho0 X), X = 3 e.u + (4.5 (e.u)^2 - 1.5 u^2)/gamma,
-// so the O(1) part rho0 w never enters single precision.
+// Equilibrium of one lattice point e at state (r, u, v, w), divided by its
+// weight.  With DEV (FP32 storage) the density is a deviation r = rho - rho0
+// and the value returned is h = (feq - w rho0) / w = r (1 + X) + rho0 X,
+// X = 3 e.u + (4.5 (e.u)^2 - 1.5 u^2)/gamma, so the O(1) part rho0 never enters
+// single precision.
 template <int EX, int EY, int EZ, bool DEV, typename T>
-__device__ __forceinline__ T pop_eq(T wk, T r, T u, T v, T w, T c45, T c15, T rho0) {
+__device__ __forceinline__ T pop_eq(T r, T u, T v, T w, T c45, T c15, T rho0) {
     const T eu = (T)EX * u + (T)EY * v + (T)EZ * w;
     const T usq = u * u + v * v + w * w;
     // X = 3 e.u + (4.5 (e.u)^2 - 1.5 u^2) / gamma, as e.u (3 + c45 e.u) - c15 u^2
     // with c45 = 4.5 / gamma and c15 = 1.5 / gamma
     const T X = eu * ((T)3 + c45 * eu) - c15 * usq;
-    if constexpr (DEV) {
-        return wk * (r + (r + rho0) * X);
-    } else {
-        const T wr = wk * r;
-        return wr + wr * X;
-    }
+    if constexpr (DEV) return r + (r + rho0) * X;
+    else return r + r * X;
 }
 
 // e.g of one lattice direction from a gradient triple g (x, y, z) and its sum
@@ -272,11 +278,11 @@ __device__ __forceinline__ T dir_deriv(const T *g, T s) {
 // `side`): for faces containing a lattice direction, e.n is zero analytically
 // and its sign is rounding noise, so evaluating it in another precision would
 // pick the other side.  The pair enters the moments through its sum
-// S = f(+e) + f(-e) (even moments) and difference D = f(+e) - f(-e) (odd ones).
+// S = h(+e) + h(-e) (even moments) and difference D = h(+e) - h(-e) (odd ones).
 template <int P, int EX, int EY, int EZ, bool DEV, typename T>
-__device__ __forceinline__ void lattice_pair(Moments<T> &M, T wk, unsigned side, T dt, const T pI[4],
-                                             const T gO[12], const T sO[4], const T pN[4], const T gN[12],
-                                             const T sN[4], T c45, T c15, T rho0) {
+__device__ __forceinline__ void lattice_pair(PairSums<T> &M, unsigned side, T dt, const T pI[4], const T gO[12],
+                                             const T sO[4], const T pN[4], const T gN[12], const T sN[4], T c45,
+                                             T c15, T rho0) {
     const bool up = (side >> (2 * P)) & 1u;      // +e point from the neighbour side
     const bool dn = (side >> (2 * P + 1)) & 1u;  // -e point from the neighbour side
     T a[4], b[4];
@@ -287,29 +293,50 @@ __device__ __forceinline__ void lattice_pair(Moments<T> &M, T wk, unsigned side,
         a[q] = up ? (pN[q] - pn * dt) : (pI[q] - po * dt);
         b[q] = dn ? (pN[q] + pn * dt) : (pI[q] + po * dt);
     }
-    const T fp = pop_eq<EX, EY, EZ, DEV, T>(wk, a[0], a[1], a[2], a[3], c45, c15, rho0);
-    const T fm = pop_eq<-EX, -EY, -EZ, DEV, T>(wk, b[0], b[1], b[2], b[3], c45, c15, rho0);
+    const T fp = pop_eq<EX, EY, EZ, DEV, T>(a[0], a[1], a[2], a[3], c45, c15, rho0);
+    const T fm = pop_eq<-EX, -EY, -EZ, DEV, T>(b[0], b[1], b[2], b[3], c45, c15, rho0);
     const T S = fp + fm, D = fp - fm;
-    M.s0 += S;
-    if (EX) M.sx += (T)EX * D;
-    if (EY) M.sy += (T)EY * D;
-    if (EZ) M.sz += (T)EZ * D;
-    if (EX) M.pxx += S;
-    if (EY) M.pyy += S;
-    if (EZ) M.pzz += S;
-    if (EX * EY) M.pxy += (T)(EX * EY) * S;
-    if (EX * EZ) M.pxz += (T)(EX * EZ) * S;
-    if (EY * EZ) M.pyz += (T)(EY * EZ) * S;
+    if constexpr (EX == 0 || EY == 0 || EZ == 0) {
+        constexpr int ax = EX ? 0 : (EY ? 1 : 2);
+        M.ax_s[ax] = S;
+        M.ax_d[ax] = D;
+    } else if constexpr (P == 3) {
+        // the first diagonal (+1, +1, +1) starts the sums
+        M.dg_s = S;
+        M.dg_x = D;
+        M.dg_y = D;
+        M.dg_z = D;
+        M.dg_xy = S;
+        M.dg_xz = S;
+        M.dg_yz = S;
+    } else {
+        M.dg_s += S;
+        M.dg_x += (T)EX * D;
+        M.dg_y += (T)EY * D;
+        M.dg_z += (T)EZ * D;
+        M.dg_xy += (T)(EX * EY) * S;
+        M.dg_xz += (T)(EX * EZ) * S;
+        M.dg_yz += (T)(EY * EZ) * S;
+    }
 }
 
-// Flux through a face of area A and unit normal n (owner -> neighbour).  With
-// DEV the momentum flux omits the constant rho0/3 n A, whose sum over a cell's
-// faces is a per-cell geometric constant added separately.
+// Reciprocal: correctly rounded in FP64; in FP32 the hardware estimate
+// (1 ulp), whose error is far below the FP32 variant's tolerance.
+template <typename T>
+__device__ __forceinline__ T lattice_rcp(T x) {
+    if constexpr (std::is_same<T, float>::value) return __builtin_amdgcn_rcpf(x);
+    else return (T)1 / x;
+}
+
+// Flux through a face of area A and unit normal n (owner -> neighbour); tau
+// is passed as its parts v3g / dt.  With DEV the momentum flux omits the
+// constant rho0/3 n A, whose sum over a cell's faces is a per-cell geometric
+// constant added separately.
 template <bool DEV, typename T>
-__device__ __forceinline__ void lattice_flux(const T n[3], T A, T dt, T tau, unsigned side, const T p0[4],
+__device__ __forceinline__ void lattice_flux(const T n[3], T A, T dt, T v3g, unsigned side, const T p0[4],
                                              const T pI[4], const T gO[12], const T pN[4], const T gN[12], T invgam,
                                              T rho0, T F[4]) {
-    Moments<T> M = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    PairSums<T> M;
     const T w0 = (T)(2.0 / 9.0), w1 = (T)(1.0 / 9.0), w7 = (T)(1.0 / 72.0);
     const T c45 = (T)4.5 * invgam, c15 = (T)1.5 * invgam;
     T sO[4], sN[4];
@@ -318,28 +345,37 @@ __device__ __forceinline__ void lattice_flux(const T n[3], T A, T dt, T tau, uns
         sO[q] = gO[3 * q] + gO[3 * q + 1] + gO[3 * q + 2];
         sN[q] = gN[3 * q] + gN[3 * q + 1] + gN[3 * q + 2];
     }
-    M.s0 = pop_eq<0, 0, 0, DEV, T>(w0, p0[0], p0[1], p0[2], p0[3], c45, c15, rho0);
-    lattice_pair<0, 1, 0, 0, DEV, T>(M, w1, side, dt, pI, gO, sO, pN, gN, sN, c45, c15, rho0);
-    lattice_pair<1, 0, 1, 0, DEV, T>(M, w1, side, dt, pI, gO, sO, pN, gN, sN, c45, c15, rho0);
-    lattice_pair<2, 0, 0, 1, DEV, T>(M, w1, side, dt, pI, gO, sO, pN, gN, sN, c45, c15, rho0);
-    lattice_pair<3, 1, 1, 1, DEV, T>(M, w7, side, dt, pI, gO, sO, pN, gN, sN, c45, c15, rho0);
-    lattice_pair<4, 1, 1, -1, DEV, T>(M, w7, side, dt, pI, gO, sO, pN, gN, sN, c45, c15, rho0);
-    lattice_pair<5, 1, -1, 1, DEV, T>(M, w7, side, dt, pI, gO, sO, pN, gN, sN, c45, c15, rho0);
-    lattice_pair<6, -1, 1, 1, DEV, T>(M, w7, side, dt, pI, gO, sO, pN, gN, sN, c45, c15, rho0);
-    const T rI = DEV ? rho0 + M.s0 : M.s0;
-    const T ir = (T)1 / rI;
-    const T ux = M.sx * ir, uy = M.sy * ir, uz = M.sz * ir;
+    const T h0 = pop_eq<0, 0, 0, DEV, T>(p0[0], p0[1], p0[2], p0[3], c45, c15, rho0);
+    lattice_pair<0, 1, 0, 0, DEV, T>(M, side, dt, pI, gO, sO, pN, gN, sN, c45, c15, rho0);
+    lattice_pair<1, 0, 1, 0, DEV, T>(M, side, dt, pI, gO, sO, pN, gN, sN, c45, c15, rho0);
+    lattice_pair<2, 0, 0, 1, DEV, T>(M, side, dt, pI, gO, sO, pN, gN, sN, c45, c15, rho0);
+    lattice_pair<3, 1, 1, 1, DEV, T>(M, side, dt, pI, gO, sO, pN, gN, sN, c45, c15, rho0);
+    lattice_pair<4, 1, 1, -1, DEV, T>(M, side, dt, pI, gO, sO, pN, gN, sN, c45, c15, rho0);
+    lattice_pair<5, 1, -1, 1, DEV, T>(M, side, dt, pI, gO, sO, pN, gN, sN, c45, c15, rho0);
+    lattice_pair<6, -1, 1, 1, DEV, T>(M, side, dt, pI, gO, sO, pN, gN, sN, c45, c15, rho0);
+    // weighted moments: S0, S1 and the diagonal of Pi
+    const T dS = w7 * M.dg_s;
+    const T s0 = w0 * h0 + w1 * (M.ax_s[0] + M.ax_s[1] + M.ax_s[2]) + dS;
+    const T sx = w1 * M.ax_d[0] + w7 * M.dg_x;
+    const T sy = w1 * M.ax_d[1] + w7 * M.dg_y;
+    const T sz = w1 * M.ax_d[2] + w7 * M.dg_z;
+    const T pxx = w1 * M.ax_s[0] + dS, pyy = w1 * M.ax_s[1] + dS, pzz = w1 * M.ax_s[2] + dS;
+    const T pxy = w7 * M.dg_xy, pxz = w7 * M.dg_xz, pyz = w7 * M.dg_yz;
+    const T tau = v3g * lattice_rcp(dt);
+    const T rI = DEV ? rho0 + s0 : s0;
+    const T ir = lattice_rcp(rI);
+    const T ux = sx * ir, uy = sy * ir, uz = sz * ir;
     const T om = (T)1 - tau;
     // sum e e^T feq(I) = rho_I (I/3 + u u^T/gamma); in DEV form rho0 I/3 is dropped
-    const T iso = DEV ? M.s0 * (T)(1.0 / 3.0) : rI * (T)(1.0 / 3.0);
+    const T iso = DEV ? s0 * (T)(1.0 / 3.0) : rI * (T)(1.0 / 3.0);
     const T k = om * rI;
-    const T mxx = om * iso + k * (ux * ux * invgam) + tau * M.pxx;
-    const T myy = om * iso + k * (uy * uy * invgam) + tau * M.pyy;
-    const T mzz = om * iso + k * (uz * uz * invgam) + tau * M.pzz;
-    const T mxy = k * (ux * uy * invgam) + tau * M.pxy;
-    const T mxz = k * (ux * uz * invgam) + tau * M.pxz;
-    const T myz = k * (uy * uz * invgam) + tau * M.pyz;
-    F[0] = (M.sx * n[0] + M.sy * n[1] + M.sz * n[2]) * A;
+    const T mxx = om * iso + k * (ux * ux * invgam) + tau * pxx;
+    const T myy = om * iso + k * (uy * uy * invgam) + tau * pyy;
+    const T mzz = om * iso + k * (uz * uz * invgam) + tau * pzz;
+    const T mxy = k * (ux * uy * invgam) + tau * pxy;
+    const T mxz = k * (ux * uz * invgam) + tau * pxz;
+    const T myz = k * (uy * uz * invgam) + tau * pyz;
+    F[0] = (sx * n[0] + sy * n[1] + sz * n[2]) * A;
     F[1] = (mxx * n[0] + mxy * n[1] + mxz * n[2]) * A;
     F[2] = (mxy * n[0] + myy * n[1] + myz * n[2]) * A;
     F[3] = (mxz * n[0] + myz * n[1] + mzz * n[2]) * A;

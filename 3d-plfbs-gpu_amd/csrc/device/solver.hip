@@ -473,9 +473,12 @@ int create_ctx(const plbfs_mesh_view *mesh, const plbfs_params *params, const pl
                                  : flux_lds_bytes<double>(L.max_items_per_block, block_cells, stage))
                        : (size_t)L.max_items_per_block * 4 * sbytes * lds_cases;
     if (c->lds_bytes > kMaxLds) {
+        // L refers into c: read what the message needs before c is freed
+        const size_t need = c->lds_bytes;
+        const int slots = L.max_items_per_block;
         delete c;
         return plbfs_fail(PLBFS_ERR_INPUT, "a block needs %zu B of LDS for %d face slots x %d cases (limit %d B)",
-                          c->lds_bytes, L.max_items_per_block, lds_cases, kMaxLds);
+                          need, slots, lds_cases, kMaxLds);
     }
     auto fail = [&](int r) {
         release(c);

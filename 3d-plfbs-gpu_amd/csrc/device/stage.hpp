@@ -325,8 +325,7 @@ __device__ __forceinline__ void item_flux(const DevMesh<S> &m, const DevParams &
 #pragma unroll
         for (int q = 0; q < 4; q++) pI[q] = pN[q] = p0[q] = fv[q];
     }
-    const S tau = (S)v3g / dt;
-    lattice_flux<std::is_same<S, float>::value, S>(n, A, dt, tau, side, p0, pI, go, pN, gN, (S)prm.invgam,
+    lattice_flux<std::is_same<S, float>::value, S>(n, A, dt, (S)v3g, side, p0, pI, go, pN, gN, (S)prm.invgam,
                                                    (S)prm.rho0, F);
 }
 

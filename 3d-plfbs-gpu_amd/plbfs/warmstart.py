@@ -395,10 +395,10 @@ def eval_main(args, params):
             c.lsq_exact_det = lsq_exact(args.eval)
         ibm = ibm_args(d, c32, args.stiffness)
         hc = mesh_for(pm, c32)
-        r_w, s_w, info = warm_solve(pm, hc, c32, ibm, net, d, args.max_steps, probe=args.probe)
+        r_w, s_w, info = warm_solve(pm, hc, c32, ibm, net, d, args.max_steps, probe=args.probe, label="warm32")
         uni = uniform_state(c32, geo, N)
-        r_u32, _ = solve_to_tolerance(pm, hc, c32, uni, ibm, args.max_steps)
-        r_u64, s_64 = solve_to_tolerance(pm, hc, c64, uni, ibm, args.max_steps)
+        r_u32, _ = solve_to_tolerance(pm, hc, c32, uni, ibm, args.max_steps, label="rest32")
+        r_u64, s_64 = solve_to_tolerance(pm, hc, c64, uni, ibm, args.max_steps, label="rest64")
         hc.close()
         uw = np.stack(s_w[1:], 1)
         u64 = np.stack(s_64[1:], 1)
@@ -411,8 +411,13 @@ def eval_main(args, params):
                                         r_u32["seconds"], r_u64["seconds"]), file=sys.stderr, flush=True)
     net.close()
     ok = [r for r in rows if r["warm_fp32"]["converged"] and r["rest_fp64"]["converged"]]
-    return dict(mesh=args.eval, cells=N, tests=rows,
-                speedup_fp64_rest_over_fp32_warm=[r["rest_fp64"]["seconds"] / r["warm_fp32"]["seconds"] for r in ok],
+    speedup = [r["rest_fp64"]["seconds"] / r["warm_fp32"]["seconds"] for r in ok]
+    err = [r["vel_rel_l2_vs_fp64"] for r in ok]
+    return dict(mesh=args.eval, cells=N, tests=rows, converged=len(ok),
+                speedup_fp64_rest_over_fp32_warm=speedup,
+                speedup_median=float(np.median(speedup)) if ok else None,
+                speedup_worst=float(np.min(speedup)) if ok else None,
+                vel_rel_l2_worst=float(np.max(err)) if ok else None,
                 steps_ratio_rest_over_warm_fp32=[r["rest_fp32"]["steps"] / max(r["warm_fp32"]["steps"], 1)
                                                  for r in ok if r["rest_fp32"]["converged"]])
 

@@ -110,9 +110,11 @@ def test_periodic_z_steps(gpu_available, tets):
 
 
 def test_flux_block_too_large_for_lds_is_a_clean_error(gpu_available, monkeypatch):
-    # 1024 FP64 tets need ~2300 face slots x 32 B > 64 KiB of LDS
+    # 1024 FP64 tets staged: ~2300 face slots x 32 B plus 1024 staged cell
+    # records x 160 B, over the 159 KiB a flux block may take; refused on the
+    # host before any launch
     monkeypatch.setenv("PLBFS_BLOCK_CELLS", "1024")
-    monkeypatch.setenv("PLBFS_FLUX_STAGE", "0")
+    monkeypatch.setenv("PLBFS_FLUX_STAGE", "1")
     pm = channel_mesh(16, 8, 8, tets=True)
     case = pc.channel_case(re=40.0, length_scale=6.0)
     hm = HostMesh.from_polymesh(pm, case.visc, case.gamma, case.fneq_min)
