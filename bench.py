@@ -282,6 +282,17 @@ def run_to_tolerance(args, emit=True):
         print("%s: %d steps, converged %s, %.1f s" % (name, r["steps"], r["converged"], r["seconds"]),
               file=sys.stderr, flush=True)
         states[name] = state
+    if getattr(args, "save_velocity", "") and "fp64" in states:
+        np.save(args.save_velocity, np.stack(states["fp64"][1:], 1).astype(np.float32))
+    if getattr(args, "ref_velocity", ""):
+        # the FP64 solution of a separate run, rounded to float32 (relative
+        # rounding 6e-8, far below the 1e-4 compared against)
+        uref = np.load(args.ref_velocity).astype(np.float64)
+        if uref.shape != (N, 3):
+            raise SystemExit("--ref-velocity: shape %s, expected (%d, 3)" % (uref.shape, N))
+        for name in states:
+            u = np.stack(states[name][1:], 1)
+            out[name]["velocity_rel_l2_vs_fp64_file"] = float(np.linalg.norm(u - uref) / np.linalg.norm(uref))
     if "fp64" in states:
         u64 = np.stack(states["fp64"][1:], 1)
         for name in states:
@@ -321,6 +332,11 @@ def main():
     ap.add_argument("--probe", type=int, default=200, help="steps of the warm-start fallback probe")
     ap.add_argument("--re", type=float, default=0.0, help="--time-to-tol: Reynolds number other than the workload's")
     ap.add_argument("--legs", default="", help="--time-to-tol: subset of fp64,fp32,fp32_warm to run")
+    ap.add_argument("--save-velocity", default="",
+                    help="--time-to-tol: write the FP64 leg's converged (u, v, w) as an N x 3 float32 .npy")
+    ap.add_argument("--ref-velocity", default="",
+                    help="--time-to-tol: FP64 velocity written by --save-velocity in an earlier run of the same "
+                         "workload; each leg's velocity rel-L2 against it is reported")
     ap.add_argument("--no-ttt", action="store_true",
                     help="throughput only: skip the time-to-1e-6 comparison on the 96x48x48 channel")
     ap.add_argument("--copy-calibration", action="store_true",
