@@ -147,14 +147,18 @@ __global__ __launch_bounds__(kBlock) void k_flux_border(DevMesh<S> m, DevParams 
     for (int q = 0; q < 4; q++) m.bflux[q * nbd + t] = F[q];
 }
 
+// FP32: 6 waves per SIMD (80 VGPRs, no spill) instead of the compiler's 5
+// (87 VGPRs) took the 4.1M-tet flux from 0.310 to 0.301 ms.  FP64 is left to
+// the compiler (143 VGPRs, 3 waves): forcing 4 waves spills 76 B per lane and
+// measured 0.684 ms against 0.543.
 template <typename S>
-__global__ __launch_bounds__(kBlock) void k_flux_update(DevMesh<S> m, DevParams prm, StageArgs sa,
-                                                        const Ctrl *__restrict__ ctrl,
-                                                        const V4<S> *__restrict__ st_in,
-                                                        const V4<S> *__restrict__ grad, V4<S> *__restrict__ st_out,
-                                                        double4 *__restrict__ t0, Acc<S> *__restrict__ t1,
-                                                        double *__restrict__ dtl, const double *__restrict__ fbody,
-                                                        double *__restrict__ partials) {
+constexpr int kFluxMinWaves = std::is_same<S, float>::value ? 6 : 1;
+
+template <typename S>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kFluxMinWaves<S>))) void k_flux_update(
+    DevMesh<S> m, DevParams prm, StageArgs sa, const Ctrl *__restrict__ ctrl, const V4<S> *__restrict__ st_in,
+    const V4<S> *__restrict__ grad, V4<S> *__restrict__ st_out, double4 *__restrict__ t0, Acc<S> *__restrict__ t1,
+    double *__restrict__ dtl, const double *__restrict__ fbody, double *__restrict__ partials) {
     extern __shared__ double lds[];
     if (ctrl->done) return;
     const int N = m.N;
