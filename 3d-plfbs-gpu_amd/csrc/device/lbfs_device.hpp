@@ -302,6 +302,7 @@ __device__ __forceinline__ void lattice_pair(PairSums<T> &M, unsigned side, T dt
         M.ax_d[ax] = D;
     } else if constexpr (P == 3) {
         // the first diagonal (+1, +1, +1) starts the sums
+        static_assert(EX == 1 && EY == 1 && EZ == 1, "pair 3 is the (+1, +1, +1) diagonal");
         M.dg_s = S;
         M.dg_x = D;
         M.dg_y = D;
