@@ -23,6 +23,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -66,6 +67,27 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
     gi[0] = V4<S>{g[0], g[1], g[2], g[3]};
     gi[1] = V4<S>{g[4], g[5], g[6], g[7]};
     gi[2] = V4<S>{g[8], g[9], g[10], g[11]};
+}
+
+// Border pass (split border): faces between two flux blocks, one thread per
+// (face, case) record with the cases of a face adjacent, evaluated once into
+// bflux [4][n_border][B] that both blocks' gathers read.
+template <typename S>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(std::is_same<S, float>::value ? 6 : 1)))
+void kb_flux_border(DevMesh<S> m, DevParams prm, BatchArgs a,
+                                                         const Ctrl *__restrict__ ctrl,
+                                                         const V4<S> *__restrict__ st_in,
+                                                         const V4<S> *__restrict__ grad) {
+    if (ctrl->done) return;
+    const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int j = (int)(k / a.B), b = (int)(k - (int64_t)j * a.B);
+    if (j >= m.n_border || a.done[b]) return;
+    const SideLoader<S, RecB, false> ld{st_in, grad, m.cgeo, RecB{a.B, b}, nullptr, nullptr, 0, 0, 1, 0};
+    S F[4];
+    item_flux<S>(m, prm, m.border0 + j, ld, a.v3g[b], a.dtcap[b], F);
+    const size_t nbd = (size_t)m.n_border;
+#pragma unroll
+    for (int q = 0; q < 4; q++) m.bflux[(q * nbd + j) * a.B + b] = F[q];
 }
 
 template <typename S>
@@ -120,7 +142,7 @@ __global__ __launch_bounds__(kBlock) void kb_flux(DevMesh<S> m, DevParams prm, B
     for (int il = threadIdx.x / C; live && il < ncell; il += kBlock / C) {
         const int i = c0 + il;
         double R[4];
-        gather_residual<S>(m, i, sF, stride, C, c, R);
+        gather_residual<S>(m, i, sF, stride, C, c, R, B, b);
         const size_t r = (size_t)i * B + b;
         if (prm.use_body_force && a.force_rec[r]) {
             R[1] += fbody[r];
@@ -271,6 +293,9 @@ struct plbfs_batch {
     // shared mesh, layout and stream
     plbfs_ctx *base = nullptr;
     int block_cells = 256, C = 1, stage = 0;
+    int split = 0;             // 1: faces between flux blocks evaluated once by kb_flux_border
+    void *bflux = nullptr;     // [4][n_border][B] border fluxes (storage precision)
+    size_t bflux_cap = 0;
     std::vector<void *> allocs;
     // device arrays over N * B records
     void *stA = nullptr, *stB = nullptr, *grad = nullptr, *t1 = nullptr;
@@ -349,7 +374,7 @@ int retile(plbfs_batch *b, int block_cells, int C, int stage) {
     plbfs_mesh_view v = view_of(b);
     plbfs_ctx *nb = nullptr;
     int rc = create_ctx(&v, &b->params, b->patches.data(), (int32_t)b->patches.size(),
-                        b->bpatch.empty() ? nullptr : b->bpatch.data(), block_cells, C, false, false, &nb);
+                        b->bpatch.empty() ? nullptr : b->bpatch.data(), block_cells, C, false, b->split != 0, &nb);
     if (rc) return rc;
     const size_t lds = lds_bytes_for(nb->L, C, b->f32, stage != 0);
     if (lds > (size_t)kMaxLds) {
@@ -368,6 +393,17 @@ int retile(plbfs_batch *b, int block_cells, int C, int stage) {
         }
         b->partials = np;
         b->partials_cap = need;
+    }
+    const size_t nbf = (size_t)4 * nb->L.n_border * b->B * (b->f32 ? 4 : 8);
+    if (nbf > b->bflux_cap) {
+        // as the partials: an array that queued work may read stays allocated
+        void *np = nullptr;
+        if ((rc = balloc(b, &np, nbf))) {
+            plbfs_destroy(nb);
+            return rc;
+        }
+        b->bflux = np;
+        b->bflux_cap = nbf;
     }
     drop_graph(b);
     if (b->base) plbfs_destroy(b->base);
@@ -489,8 +525,12 @@ int ibm_upload(plbfs_batch *b) {
 }
 
 template <typename S>
-void launch_batch_stages(plbfs_batch *b, const DevMesh<S> &dm) {
+void launch_batch_stages(plbfs_batch *b, const DevMesh<S> &dm_base) {
     plbfs_ctx *c = b->base;
+    // the base context's border array holds one case; the batch's holds B
+    DevMesh<S> dm = dm_base;
+    dm.bflux = (S *)b->bflux;
+    const int64_t nbr = (int64_t)dm.n_border * b->B;
     const BatchArgs a = args_of(b);
     const int64_t NR = (int64_t)b->N * b->B;
     const dim3 rgrid((unsigned)((NR + kBlock - 1) / kBlock));
@@ -504,6 +544,9 @@ void launch_batch_stages(plbfs_batch *b, const DevMesh<S> &dm) {
         hipLaunchKernelGGL(kb_grad<S>, rgrid, dim3(kBlock), 0, c->stream, dm, a, b->ctrl, (const V4<S> *)in,
                            (V4<S> *)b->grad);
         StageArgs sa{kAl[k], kBe[k], k};
+        if (nbr > 0)
+            hipLaunchKernelGGL(kb_flux_border<S>, dim3((unsigned)((nbr + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                               c->stream, dm, prm, a, b->ctrl, (const V4<S> *)in, (const V4<S> *)b->grad);
         hipLaunchKernelGGL(kb_flux<S>, fgrid, dim3(kBlock), c->lds_bytes, c->stream, dm, prm, a, sa, b->ctrl,
                            (const V4<S> *)in, (const V4<S> *)b->grad, (V4<S> *)out, (double4 *)b->t0,
                            (Acc<S> *)b->t1, b->dtl, b->fbody, b->partials);
@@ -646,6 +689,7 @@ extern "C" int plbfs_batch_create(const plbfs_mesh_view *mesh, const plbfs_param
     b->visc.assign(case_visc, case_visc + n_cases);
     b->f32 = params->precision == PLBFS_FP32;
     b->ibm_case.resize(n_cases);
+    if (const char *sp = std::getenv("PLBFS_BATCH_SPLIT_BORDER")) b->split = std::atoi(sp) != 0;
     auto fail = [&](int rc) {
         plbfs_batch_destroy(b);
         return rc;

@@ -333,7 +333,7 @@ __device__ __forceinline__ void item_flux(const DevMesh<S> &m, const DevParams &
 // order (src/Solver.cpp:353-366).  Slot sl of lane c lives at sF[q*stride + sl*C + c].
 template <typename S>
 __device__ __forceinline__ void gather_residual(const DevMesh<S> &m, int i, const S *sF, int stride, int C, int c,
-                                                double R[4]) {
+                                                double R[4], int B = 1, int b = 0) {
     const double iv = m.inv_vol[i];
     R[0] = R[1] = R[2] = R[3] = 0;
     const int k0 = m.g_off[i], k1 = m.g_off[i + 1];
@@ -341,10 +341,11 @@ __device__ __forceinline__ void gather_residual(const DevMesh<S> &m, int i, cons
         const int ent = m.g_ent[k];
         S v[4];
         if (ent & 2) {
-            // a face between two blocks: its flux was evaluated once by the border pass
+            // a face between two blocks: its flux was evaluated once by the border
+            // pass, [4][n_border][B] with the case fastest (B = 1 for one case)
             const size_t j = (size_t)(ent >> 2), nbd = (size_t)m.n_border;
 #pragma unroll
-            for (int q = 0; q < 4; q++) v[q] = m.bflux[q * nbd + j];
+            for (int q = 0; q < 4; q++) v[q] = m.bflux[(q * nbd + j) * B + b];
         } else {
             const int sl = (ent >> 2) * C + c;
 #pragma unroll

@@ -151,6 +151,33 @@ def test_batch_per_case_spheres_equal_single_runs(gpu_available):
             assert np.array_equal(bt.state(b)[q], s1[q]), (b, q, rel_err(bt.state(b)[q], s1[q]))
 
 
+@pytest.mark.parametrize("precision,tiling", [(1, (64, 4, 0)), (1, (64, 8, 0)), (1, (128, 2, 1)), (0, (64, 2, 0))])
+def test_batch_border_pass_does_not_change_fields(gpu_available, monkeypatch, precision, tiling):
+    # PLBFS_BATCH_SPLIT_BORDER=1: faces between two flux blocks are evaluated
+    # once per case by kb_flux_border instead of once by each block; the same
+    # face arithmetic, so every case's fields stay bit for bit the same
+    pm = channel_mesh(12, 6, 6, tets=True)
+    res = [25.0, 50.0, 75.0, 100.0, 150.0, 200.0, 40.0, 80.0]
+    cases = cases_for(res, precision=precision, fneq_min=0.55)
+    for c in cases:
+        c.lsq_exact_det = 1
+    o = oracle_for(pm, cases[0])
+    st = perturbed_state(cases[0], o, amp=0.01, seed=9)
+    outs = []
+    for split in ("0", "1"):
+        monkeypatch.setenv("PLBFS_BATCH_SPLIT_BORDER", split)
+        hm, bt = make_batch(pm, cases)
+        bt.set_tiling(*tiling)
+        bt.set_state(-1, *st)
+        rc, r = bt.run(25)
+        assert rc == 0 and all(x["steps"] == 25 for x in r), r
+        outs.append([bt.state(b) for b in range(len(cases))])
+        bt.close()
+    for b in range(len(cases)):
+        for q in range(4):
+            assert np.array_equal(outs[0][b][q], outs[1][b][q]), (b, q, rel_err(outs[1][b][q], outs[0][b][q]))
+
+
 def test_autotune_keeps_results(gpu_available):
     pm = channel_mesh(12, 6, 6)
     cases = cases_for([30.0, 60.0, 90.0, 120.0], precision=1)
