@@ -564,7 +564,7 @@ int launch_batch_step(plbfs_batch *b) {
         else
             hipLaunchKernelGGL(k_ibm_nodes<double4>, grid, dim3(kIbmThreads), 0, c->stream, b->ia, b->ctrl,
                                (const double4 *)b->stA);
-        hipLaunchKernelGGL(k_ibm_spread, dim3((b->ia.n_affected + kNodesPerBlock - 1) / kNodesPerBlock),
+        hipLaunchKernelGGL(k_ibm_spread, dim3((b->ia.n_affected + kSpreadPerBlock - 1) / kSpreadPerBlock),
                            dim3(kIbmThreads), 0, c->stream, b->ia, b->ctrl, b->fbody);
     }
     if (b->f32) launch_batch_stages<float>(b, c->dm32);

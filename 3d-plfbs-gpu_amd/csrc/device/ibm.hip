@@ -78,7 +78,7 @@ int ibm_launch(plbfs_ctx *c, Ctrl *ctrl, bool timed, bool rebuild) {
     else
         hipLaunchKernelGGL(k_ibm_nodes<double4>, grid, dim3(kIbmThreads), 0, c->stream, a, ctrl,
                            (const double4 *)c->stA);
-    hipLaunchKernelGGL(k_ibm_spread, dim3((a.n_affected + kNodesPerBlock - 1) / kNodesPerBlock), dim3(kIbmThreads), 0, c->stream, a,
+    hipLaunchKernelGGL(k_ibm_spread, dim3((a.n_affected + kSpreadPerBlock - 1) / kSpreadPerBlock), dim3(kIbmThreads), 0, c->stream, a,
                        ctrl, c->fbody);
     HIP_TRY(hipGetLastError());
     return PLBFS_OK;

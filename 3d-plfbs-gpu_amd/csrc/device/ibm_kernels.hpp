@@ -6,7 +6,7 @@
 //                 from the step-start state (src/immersed_boundary_method.cu:13-90),
 //                 move X = X0 + U dt (:244-295), spring force F = -k dA (X - X_ref)
 //                 (:161-190), and the spreading normaliser at the new position;
-//   k_ibm_spread  one wavefront per affected (cell, case) record: sum of the
+//   k_ibm_spread  16 lanes per affected (cell, case) record: sum of the
 //                 node forces over its pairs (:299-375).
 // Weights are normalised per node, W_cn = delta V_c / sum delta V, which is the
 // reference's formula on its uniform unit grid.  No atomics.
@@ -151,21 +151,36 @@ __global__ __launch_bounds__(kIbmThreads) void k_ibm_weights(IbmArgs a) {
         a.pw[p] = ibm_delta(a.kernel, a.ih, a.cc, a.N, a.np_cell[p], X);
 }
 
-// One 64-wide wavefront per affected record; lanes stride over its pairs in
-// node order and a fixed butterfly sums them.  fbody is [3][B*N]
-// component-major over records.
+// kSpreadLanes lanes per affected record (kSpreadPerBlock records per
+// workgroup): the lanes stride over the record's pairs in node order and a fixed
+// butterfly inside the lane group sums them, so every record's force has one
+// summation order wherever it is spread from.  A record has a few dozen pairs
+// (a node's Peskin-4 support spans 4^3 cells, its candidate list with the drift
+// margin 6^3), so a whole wavefront per record would leave most lanes idle.
+// fbody is [3][B*N] component-major over records.
+constexpr int kSpreadLanes = 16;
+constexpr int kSpreadPerBlock = kIbmThreads / kSpreadLanes;
+
+__device__ __forceinline__ double group_allsum(double x) {
+#pragma unroll
+    for (int off = kSpreadLanes / 2; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    return x;
+}
+
 __global__ __launch_bounds__(kIbmThreads) void k_ibm_spread(IbmArgs a, const Ctrl *__restrict__ ctrl,
                                                             double *__restrict__ fbody) {
     if (ctrl->done) return;
-    const int lane = threadIdx.x & 63;
-    const int j = blockIdx.x * kNodesPerBlock + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & (kSpreadLanes - 1);
+    const int j = blockIdx.x * kSpreadPerBlock + (int)(threadIdx.x / kSpreadLanes);
+    // every lane of a group has the same j and case, so a group leaves as a
+    // whole and the shuffles below never read a lane that has left
     if (j >= a.n_affected) return;
     const int c = a.cp_cell[j];
     const int b = a.cp_case ? a.cp_case[j] : 0;
     if (a.case_done && a.case_done[b]) return;
     const int n = a.n;
     double f0 = 0, f1 = 0, f2 = 0;
-    for (int p = a.cp_off[j] + lane; p < a.cp_off[j + 1]; p += 64) {
+    for (int p = a.cp_off[j] + lane; p < a.cp_off[j + 1]; p += kSpreadLanes) {
         const int k = a.cp_node[p];
         const double d = a.pw[a.cp_pair[p]];  // at the node's new position
         if (d == 0.0) continue;
@@ -174,9 +189,9 @@ __global__ __launch_bounds__(kIbmThreads) void k_ibm_spread(IbmArgs a, const Ctr
         f1 += a.force[(size_t)n + k] * q;
         f2 += a.force[2 * (size_t)n + k] * q;
     }
-    f0 = wave_allsum(f0);
-    f1 = wave_allsum(f1);
-    f2 = wave_allsum(f2);
+    f0 = group_allsum(f0);
+    f1 = group_allsum(f1);
+    f2 = group_allsum(f2);
     if (lane == 0) {
         const size_t NR = (size_t)a.N * a.B, r = (size_t)c * a.B + b;
         fbody[r] = f0;
