@@ -23,6 +23,7 @@ namespace {
 
 constexpr int kIbmThreads = 256;
 constexpr int kNodesPerBlock = kIbmThreads / 64;
+constexpr int kIbmUnroll = 4;  // pairs per lane whose loads are issued together
 
 __device__ __forceinline__ double ibm_phi(int kernel, double r) {
     r = fabs(r);
@@ -78,17 +79,36 @@ __global__ __launch_bounds__(kIbmThreads) void k_ibm_nodes(IbmArgs a, Ctrl *__re
     const double X[3] = {a.x[k], a.x[(size_t)n + k], a.x[2 * (size_t)n + k]};
     const int p0 = a.np_off[k], p1 = a.np_off[k + 1];
     double S = 0, su = 0, sv = 0, sw = 0;
-    // weights at X: computed at set-up or by the previous step's second pass
-    for (int p = p0 + lane; p < p1; p += 64) {
-        const int c = a.np_cell[p];
-        const double d = a.pw[p];
-        if (d != 0.0) {
-            const double wv = d * a.vol[c];
-            const SV s = st[(size_t)c * a.B + b];
-            S += wv;
-            su += wv * s.y;
-            sv += wv * s.z;
-            sw += wv * s.w;
+    // weights at X: computed at set-up or by the previous step's second pass.
+    // A lane takes kIbmUnroll of its pairs at a time and issues all their
+    // loads before the first use, so their latencies overlap; the sums still
+    // run over the lane's pairs in list order.
+    for (int pb = p0 + lane; pb < p1; pb += 64 * kIbmUnroll) {
+        int c[kIbmUnroll];
+        double d[kIbmUnroll], vol[kIbmUnroll];
+        SV sc[kIbmUnroll];
+#pragma unroll
+        for (int u = 0; u < kIbmUnroll; u++) {
+            const int p = pb + 64 * u;
+            c[u] = p < p1 ? a.np_cell[p] : 0;
+            d[u] = p < p1 ? a.pw[p] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < kIbmUnroll; u++) {
+            if (d[u] != 0.0) {
+                vol[u] = a.vol[c[u]];
+                sc[u] = st[(size_t)c[u] * a.B + b];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kIbmUnroll; u++) {
+            if (d[u] != 0.0) {
+                const double wv = d[u] * vol[u];
+                S += wv;
+                su += wv * sc[u].y;
+                sv += wv * sc[u].z;
+                sw += wv * sc[u].w;
+            }
         }
     }
     S = wave_allsum(S);
@@ -109,11 +129,26 @@ __global__ __launch_bounds__(kIbmThreads) void k_ibm_nodes(IbmArgs a, Ctrl *__re
         drift = fmax(drift, fabs(Xn[q] - a.xc[(size_t)q * n + k]));
     }
     double S2 = 0;
-    for (int p = p0 + lane; p < p1; p += 64) {
-        const int c = a.np_cell[p];
-        const double d = ibm_delta(a.kernel, a.ih, a.cc, a.N, c, Xn);
-        a.pw[p] = d;
-        if (d != 0.0) S2 += d * a.vol[c];
+    for (int pb = p0 + lane; pb < p1; pb += 64 * kIbmUnroll) {
+        int c[kIbmUnroll];
+        double d[kIbmUnroll], vol[kIbmUnroll];
+#pragma unroll
+        for (int u = 0; u < kIbmUnroll; u++) c[u] = pb + 64 * u < p1 ? a.np_cell[pb + 64 * u] : 0;
+#pragma unroll
+        for (int u = 0; u < kIbmUnroll; u++) {
+            const int p = pb + 64 * u;
+            d[u] = 0.0;
+            if (p < p1) {
+                d[u] = ibm_delta(a.kernel, a.ih, a.cc, a.N, c[u], Xn);
+                a.pw[p] = d[u];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kIbmUnroll; u++)
+            if (d[u] != 0.0) vol[u] = a.vol[c[u]];
+#pragma unroll
+        for (int u = 0; u < kIbmUnroll; u++)
+            if (d[u] != 0.0) S2 += d[u] * vol[u];
     }
     S2 = wave_allsum(S2);
     if (lane == 0) {
@@ -180,14 +215,35 @@ __global__ __launch_bounds__(kIbmThreads) void k_ibm_spread(IbmArgs a, const Ctr
     if (a.case_done && a.case_done[b]) return;
     const int n = a.n;
     double f0 = 0, f1 = 0, f2 = 0;
-    for (int p = a.cp_off[j] + lane; p < a.cp_off[j + 1]; p += kSpreadLanes) {
-        const int k = a.cp_node[p];
-        const double d = a.pw[a.cp_pair[p]];  // at the node's new position
-        if (d == 0.0) continue;
-        const double q = d / a.norm[k];
-        f0 += a.force[k] * q;
-        f1 += a.force[(size_t)n + k] * q;
-        f2 += a.force[2 * (size_t)n + k] * q;
+    const int q0 = a.cp_off[j], q1 = a.cp_off[j + 1];
+    // kIbmUnroll pairs per lane at a time, loads first (as in k_ibm_nodes)
+    for (int pb = q0 + lane; pb < q1; pb += kSpreadLanes * kIbmUnroll) {
+        int k[kIbmUnroll];
+        double d[kIbmUnroll], nk[kIbmUnroll], fx[kIbmUnroll], fy[kIbmUnroll], fz[kIbmUnroll];
+#pragma unroll
+        for (int u = 0; u < kIbmUnroll; u++) {
+            const int p = pb + kSpreadLanes * u;
+            k[u] = p < q1 ? a.cp_node[p] : 0;
+            d[u] = p < q1 ? a.pw[a.cp_pair[p]] : 0.0;  // at the node's new position
+        }
+#pragma unroll
+        for (int u = 0; u < kIbmUnroll; u++) {
+            if (d[u] != 0.0) {
+                nk[u] = a.norm[k[u]];
+                fx[u] = a.force[k[u]];
+                fy[u] = a.force[(size_t)n + k[u]];
+                fz[u] = a.force[2 * (size_t)n + k[u]];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kIbmUnroll; u++) {
+            if (d[u] != 0.0) {
+                const double q = d[u] / nk[u];
+                f0 += fx[u] * q;
+                f1 += fy[u] * q;
+                f2 += fz[u] * q;
+            }
+        }
     }
     f0 = group_allsum(f0);
     f1 = group_allsum(f1);
