@@ -74,10 +74,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
 // bflux [4][n_border][B] that both blocks' gathers read.
 template <typename S>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(std::is_same<S, float>::value ? 6 : 1)))
-void kb_flux_border(DevMesh<S> m, DevParams prm, BatchArgs a,
-                                                         const Ctrl *__restrict__ ctrl,
-                                                         const V4<S> *__restrict__ st_in,
-                                                         const V4<S> *__restrict__ grad) {
+void kb_flux_border(DevMesh<S> m, DevParams prm, BatchArgs a, const Ctrl *__restrict__ ctrl,
+                    const V4<S> *__restrict__ st_in, const V4<S> *__restrict__ grad) {
     if (ctrl->done) return;
     const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     const int j = (int)(k / a.B), b = (int)(k - (int64_t)j * a.B);
