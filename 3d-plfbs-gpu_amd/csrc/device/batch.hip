@@ -184,10 +184,26 @@ __global__ __launch_bounds__(kFinThreads) void kb_finalize(int nblk, int B, doub
     if (cc.done[b]) return;
     __shared__ double red[kFinThreads][5];
     double s[5] = {0, 0, 0, 0, 0};
-    for (int k = threadIdx.x; k < nblk; k += kFinThreads) {
-        const double *p = partials + ((size_t)k * B + b) * 5;
-        for (int c = 0; c < 4; c++) s[c] += p[c];
-        s[4] = fmax(s[4], p[4]);
+    // a thread's blocks k = tid, tid + T, ... in that order; four at a time
+    // with all their loads issued before the sums
+    constexpr int kU = 4;
+    for (int k0 = threadIdx.x; k0 < nblk; k0 += kFinThreads * kU) {
+        double v[kU][5];
+#pragma unroll
+        for (int u = 0; u < kU; u++) {
+            const int k = k0 + u * kFinThreads;
+            if (k < nblk) {
+                const double *p = partials + ((size_t)k * B + b) * 5;
+                for (int c = 0; c < 5; c++) v[u][c] = p[c];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kU; u++) {
+            if (k0 + u * kFinThreads < nblk) {
+                for (int c = 0; c < 4; c++) s[c] += v[u][c];
+                s[4] = fmax(s[4], v[u][4]);
+            }
+        }
     }
     for (int c = 0; c < 5; c++) red[threadIdx.x][c] = s[c];
     __syncthreads();

@@ -271,9 +271,24 @@ __global__ __launch_bounds__(kFinalizeThreads) void k_finalize(int nblk, double 
     if (ctrl->done) return;
     __shared__ double red[kFinalizeThreads][5];
     double s[5] = {0, 0, 0, 0, 0};
-    for (int b = threadIdx.x; b < nblk; b += kFinalizeThreads) {
-        for (int c = 0; c < 4; c++) s[c] += partials[(size_t)b * 5 + c];
-        s[4] = fmax(s[4], partials[(size_t)b * 5 + 4]);
+    // a thread's partials b = tid, tid + T, ... in that order; four at a time
+    // with all their loads issued before the sums
+    constexpr int kU = 4;
+    for (int b0 = threadIdx.x; b0 < nblk; b0 += kFinalizeThreads * kU) {
+        double v[kU][5];
+#pragma unroll
+        for (int u = 0; u < kU; u++) {
+            const int b = b0 + u * kFinalizeThreads;
+            if (b < nblk)
+                for (int c = 0; c < 5; c++) v[u][c] = partials[(size_t)b * 5 + c];
+        }
+#pragma unroll
+        for (int u = 0; u < kU; u++) {
+            if (b0 + u * kFinalizeThreads < nblk) {
+                for (int c = 0; c < 4; c++) s[c] += v[u][c];
+                s[4] = fmax(s[4], v[u][4]);
+            }
+        }
     }
     for (int c = 0; c < 5; c++) red[threadIdx.x][c] = s[c];
     __syncthreads();
